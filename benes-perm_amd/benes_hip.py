@@ -1,0 +1,257 @@
+# SPDX-License-Identifier: Apache-2.0
+"""ctypes binding of the gfx950 compression backend (include/benes_hip.h).
+
+Mirrors the reference's compression interface: ``compress(permutation,
+format)`` returns the compact version-1 blob of a uint32/uint64 permutation,
+``FormatOptions`` carries ``middle_group_log2``, the two cluster lists and the
+middle placement (include/benes/benes.hpp:32-39 of the reference), and errors
+are raised as the Python analogue of the reference's exception class:
+ValueError for std::invalid_argument, OverflowError for std::overflow_error,
+MemoryError for std::length_error, AssertionError for std::logic_error and
+RuntimeError for std::runtime_error.
+
+The shared library must have been built (``make -C benes-perm_amd``); there is
+no fallback of any kind.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+from typing import Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbenes_hip.so")
+
+OK, INVALID_ARGUMENT, OVERFLOW, LENGTH, LOGIC, RUNTIME, BUFFER_TOO_SMALL = range(7)
+
+SEPARATE, EMBEDDED_INPUT, EMBEDDED_OUTPUT = 0, 1, 2
+FISHER_YATES, SINGLE_CYCLE, NEAR_IDENTITY, IDENTITY, REVERSE = range(5)
+
+# Every extern "C" symbol declared in include/benes_hip.h.
+EXPORTS = (
+    "benes_hip_available",
+    "benes_hip_compress",
+    "benes_hip_compress_device",
+    "benes_hip_encoded_size",
+    "benes_hip_last_device_seconds",
+    "benes_hip_set_profiling",
+    "benes_hip_kernel_stats",
+    "benes_hip_layout_of",
+    "benes_hip_node_address",
+    "benes_hip_address_bit",
+    "benes_hip_generate",
+)
+
+
+class _Format(ctypes.Structure):
+    _fields_ = [
+        ("middle_group_log2", ctypes.c_uint8),
+        ("middle_placement", ctypes.c_uint8),
+        ("n_input_clusters", ctypes.c_uint8),
+        ("n_output_clusters", ctypes.c_uint8),
+        ("input_clusters", ctypes.POINTER(ctypes.c_uint8)),
+        ("output_clusters", ctypes.POINTER(ctypes.c_uint8)),
+    ]
+
+
+class _Layout(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_uint64),
+        ("network_size", ctypes.c_uint64),
+        ("input_offset", ctypes.c_uint64),
+        ("output_offset", ctypes.c_uint64),
+        ("middle_offset", ctypes.c_uint64),
+        ("end_offset", ctypes.c_uint64),
+        ("switch_bits", ctypes.c_uint64),
+        ("middle_bits", ctypes.c_uint64),
+        ("routing_depth", ctypes.c_uint32),
+        ("middle_group_log2", ctypes.c_uint8),
+        ("middle_placement", ctypes.c_uint8),
+        ("n_input_clusters", ctypes.c_uint8),
+        ("n_output_clusters", ctypes.c_uint8),
+        ("input_clusters", ctypes.c_uint8 * 64),
+        ("output_clusters", ctypes.c_uint8 * 64),
+    ]
+
+
+class _Address(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "count", "packet_base", "full_packet_bits", "full_packet_prefix",
+        "tail_packet_prefix", "pair_span", "tail_packet", "rank_bit", "offset")]
+
+
+@dataclasses.dataclass
+class FormatOptions:
+    """The reference's FormatOptions (include/benes/benes.hpp:32-39)."""
+    middle_group_log2: int = 5
+    input_clusters: Sequence[int] = ()
+    output_clusters: Sequence[int] = ()
+    middle_placement: int = SEPARATE
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Loads libbenes_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `make -C benes-perm_amd`")
+    lib = ctypes.CDLL(LIB_PATH)
+    u64, u32, vp, cp, sz = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t
+    fp = ctypes.POINTER(_Format)
+    lib.benes_hip_available.restype = ctypes.c_int
+    lib.benes_hip_compress.argtypes = [vp, u64, u32, fp, vp, u64, ctypes.POINTER(u64),
+                                       ctypes.POINTER(ctypes.c_double), cp, sz]
+    lib.benes_hip_compress_device.argtypes = [vp, u64, u32, fp, vp, u64, ctypes.POINTER(u64),
+                                              vp, cp, sz]
+    lib.benes_hip_encoded_size.argtypes = [u64, fp, ctypes.POINTER(u64), cp, sz]
+    lib.benes_hip_last_device_seconds.restype = ctypes.c_double
+    lib.benes_hip_set_profiling.argtypes = [ctypes.c_int]
+    lib.benes_hip_set_profiling.restype = None
+    lib.benes_hip_kernel_stats.argtypes = [ctypes.c_int, cp, sz, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(u64)]
+    lib.benes_hip_layout_of.argtypes = [u64, fp, ctypes.POINTER(_Layout), cp, sz]
+    lib.benes_hip_node_address.argtypes = [u64, fp, ctypes.c_int, u32, u64,
+                                           ctypes.POINTER(_Address), cp, sz]
+    lib.benes_hip_address_bit.argtypes = [ctypes.POINTER(_Address), u64]
+    lib.benes_hip_address_bit.restype = u64
+    lib.benes_hip_generate.argtypes = [u32, u64, u64, u32, vp, u32]
+    _lib = lib
+    return lib
+
+
+_ERRORS = {
+    INVALID_ARGUMENT: ValueError,
+    OVERFLOW: OverflowError,
+    LENGTH: MemoryError,
+    LOGIC: AssertionError,
+    RUNTIME: RuntimeError,
+    BUFFER_TOO_SMALL: BufferError,
+}
+
+
+def _raise(rc: int, err: ctypes.Array) -> None:
+    if rc != OK:
+        raise _ERRORS.get(rc, RuntimeError)(err.value.decode(errors="replace"))
+
+
+def _format(fmt: FormatOptions | None):
+    fmt = fmt or FormatOptions()
+    ic = (ctypes.c_uint8 * max(1, len(fmt.input_clusters)))(*fmt.input_clusters)
+    oc = (ctypes.c_uint8 * max(1, len(fmt.output_clusters)))(*fmt.output_clusters)
+    f = _Format(fmt.middle_group_log2 & 0xFF, fmt.middle_placement & 0xFF,
+                len(fmt.input_clusters), len(fmt.output_clusters),
+                ctypes.cast(ic, ctypes.POINTER(ctypes.c_uint8)),
+                ctypes.cast(oc, ctypes.POINTER(ctypes.c_uint8)))
+    return f, (ic, oc)
+
+
+def available() -> bool:
+    return bool(library().benes_hip_available())
+
+
+def encoded_size(n: int, fmt: FormatOptions | None = None) -> int:
+    f, keep = _format(fmt)
+    out = ctypes.c_uint64()
+    err = ctypes.create_string_buffer(256)
+    _raise(library().benes_hip_encoded_size(n, ctypes.byref(f), ctypes.byref(out), err, 256), err)
+    return out.value
+
+
+def compress(perm: np.ndarray, fmt: FormatOptions | None = None) -> bytes:
+    """Compresses a host uint32/uint64 permutation on the GPU."""
+    perm = np.ascontiguousarray(perm)
+    if perm.dtype not in (np.uint32, np.uint64):
+        raise TypeError("permutation must be uint32 or uint64")
+    f, keep = _format(fmt)
+    lib = library()
+    err = ctypes.create_string_buffer(256)
+    size = ctypes.c_uint64()
+    rc = lib.benes_hip_compress(perm.ctypes.data_as(ctypes.c_void_p), perm.size,
+                                perm.dtype.itemsize, ctypes.byref(f), None, 0,
+                                ctypes.byref(size), None, err, 256)
+    if rc != BUFFER_TOO_SMALL:
+        _raise(rc, err)
+    out = np.empty(size.value, np.uint8)
+    secs = ctypes.c_double()
+    rc = lib.benes_hip_compress(perm.ctypes.data_as(ctypes.c_void_p), perm.size,
+                                perm.dtype.itemsize, ctypes.byref(f),
+                                out.ctypes.data_as(ctypes.c_void_p), out.size,
+                                ctypes.byref(size), ctypes.byref(secs), err, 256)
+    _raise(rc, err)
+    return out.tobytes()
+
+
+def compress_device(d_perm_ptr: int, n: int, width: int, d_out_ptr: int, capacity: int,
+                    fmt: FormatOptions | None = None, stream: int | None = None) -> int:
+    """Device-resident compress: pointers are device addresses.  Returns the
+    blob size."""
+    f, keep = _format(fmt)
+    err = ctypes.create_string_buffer(256)
+    size = ctypes.c_uint64()
+    rc = library().benes_hip_compress_device(ctypes.c_void_p(d_perm_ptr), n, width,
+                                             ctypes.byref(f), ctypes.c_void_p(d_out_ptr),
+                                             capacity, ctypes.byref(size),
+                                             ctypes.c_void_p(stream or 0), err, 256)
+    _raise(rc, err)
+    return size.value
+
+
+def last_device_seconds() -> float:
+    return library().benes_hip_last_device_seconds()
+
+
+def set_profiling(enable: bool) -> None:
+    """Per-kernel HIP-event timing of later calls (clears the sums)."""
+    library().benes_hip_set_profiling(1 if enable else 0)
+
+
+def kernel_stats() -> dict:
+    """{kernel name: (total ms, launches)} since profiling was enabled."""
+    lib = library()
+    out = {}
+    name = ctypes.create_string_buffer(64)
+    ms = ctypes.c_double()
+    launches = ctypes.c_uint64()
+    i = 0
+    while lib.benes_hip_kernel_stats(i, name, 64, ctypes.byref(ms), ctypes.byref(launches)):
+        out[name.value.decode()] = (ms.value, launches.value)
+        i += 1
+    return out
+
+
+def layout_of(n: int, fmt: FormatOptions | None = None) -> dict:
+    f, keep = _format(fmt)
+    out = _Layout()
+    err = ctypes.create_string_buffer(256)
+    _raise(library().benes_hip_layout_of(n, ctypes.byref(f), ctypes.byref(out), err, 256), err)
+    d = {name: getattr(out, name) for name, _ in _Layout._fields_
+         if name not in ("input_clusters", "output_clusters")}
+    d["input_clusters"] = list(out.input_clusters[:out.n_input_clusters])
+    d["output_clusters"] = list(out.output_clusters[:out.n_output_clusters])
+    return d
+
+
+def node_address(n: int, fmt: FormatOptions | None, side: int, depth: int, path: int) -> dict:
+    f, keep = _format(fmt)
+    out = _Address()
+    err = ctypes.create_string_buffer(256)
+    _raise(library().benes_hip_node_address(n, ctypes.byref(f), side, depth, path,
+                                            ctypes.byref(out), err, 256), err)
+    return {name: getattr(out, name) for name, _ in _Address._fields_}
+
+
+def generate(kind: int, n: int, seed: int, rate_ppm: int = 0, width: int = 4) -> np.ndarray:
+    """Seeded synthetic permutation (splitmix64 + Lemire bounded draws)."""
+    out = np.empty(n, np.uint32 if width == 4 else np.uint64)
+    rc = library().benes_hip_generate(kind, n, seed, rate_ppm,
+                                      out.ctypes.data_as(ctypes.c_void_p), width)
+    if rc != OK:
+        raise ValueError("invalid generator arguments")
+    return out

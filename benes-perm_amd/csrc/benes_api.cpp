@@ -1,0 +1,109 @@
+// SPDX-License-Identifier: Apache-2.0
+// C++ layer over the C ABI: turns status codes back into the exception types
+// the reference throws (src/benes.cpp, src/internal.hpp) and returns blobs by
+// value, as detail::compress_compact_cuda does (src/internal.hpp:1193-1198).
+#include <benes/benes.hpp>
+
+#include <stdexcept>
+#include <string>
+
+#include "benes_hip.h"
+
+namespace benes {
+namespace {
+
+[[noreturn]] void rethrow(int code, const char* message) {
+    switch (code) {
+    case BENES_HIP_INVALID_ARGUMENT: throw std::invalid_argument(message);
+    case BENES_HIP_OVERFLOW: throw std::overflow_error(message);
+    case BENES_HIP_LENGTH: throw std::length_error(message);
+    case BENES_HIP_LOGIC: throw std::logic_error(message);
+    default: throw std::runtime_error(message);
+    }
+}
+
+template <typename Value>
+std::vector<std::byte> compress_hip(std::span<const Value> permutation,
+                                    const FormatOptions& format) {
+    benes_hip_format f{};
+    f.middle_group_log2 = format.middle_group_log2;
+    f.middle_placement = static_cast<std::uint8_t>(format.middle_placement);
+    if (format.input_clusters.size() > 255 || format.output_clusters.size() > 255) {
+        throw std::invalid_argument("cluster depths do not span every routing level");
+    }
+    f.n_input_clusters = static_cast<std::uint8_t>(format.input_clusters.size());
+    f.n_output_clusters = static_cast<std::uint8_t>(format.output_clusters.size());
+    f.input_clusters = format.input_clusters.data();
+    f.output_clusters = format.output_clusters.data();
+    char err[256] = {0};
+    std::uint64_t size = 0;
+    // Size query first: errors in the input size or format surface here.
+    int rc = permutation.empty()
+        ? BENES_HIP_INVALID_ARGUMENT
+        : benes_hip_encoded_size(permutation.size(), &f, &size, err, sizeof(err));
+    if (permutation.empty()) rethrow(rc, "permutation size is out of range");
+    if (rc != BENES_HIP_OK) rethrow(rc, err);
+    std::vector<std::byte> out(static_cast<std::size_t>(size));
+    rc = benes_hip_compress(permutation.data(), permutation.size(), sizeof(Value), &f,
+                            reinterpret_cast<std::uint8_t*>(out.data()), out.size(), &size,
+                            nullptr, err, sizeof(err));
+    if (rc != BENES_HIP_OK) rethrow(rc, err);
+    return out;
+}
+
+template <typename Value>
+std::vector<std::byte> dispatch(std::span<const Value> permutation, CompressOptions options) {
+    Backend backend = options.backend;
+    if (backend == Backend::automatic) backend = Backend::hip;
+    if (backend != Backend::hip || !detail::hip_available()) {
+        throw std::runtime_error(std::string(backend_name(backend)) + " backend is unavailable");
+    }
+    return compress_hip(permutation, options.format);
+}
+
+}  // namespace
+
+std::vector<std::byte> compress(std::span<const std::uint32_t> permutation,
+                                CompressOptions options) {
+    return dispatch(permutation, std::move(options));
+}
+
+std::vector<std::byte> compress(std::span<const std::uint64_t> permutation,
+                                CompressOptions options) {
+    return dispatch(permutation, std::move(options));
+}
+
+bool backend_available(Backend backend) noexcept {
+    return (backend == Backend::automatic || backend == Backend::hip) && detail::hip_available();
+}
+
+const char* backend_name(Backend backend) noexcept {
+    switch (backend) {
+    case Backend::automatic: return "automatic";
+    case Backend::portable_cpu: return "portable-cpu";
+    case Backend::metal: return "metal";
+    case Backend::cuda: return "cuda";
+    case Backend::avx512: return "avx512";
+    case Backend::hip: return "hip";
+    }
+    return "unknown";
+}
+
+namespace detail {
+
+bool hip_available() noexcept { return benes_hip_available() != 0; }
+
+std::vector<std::byte> compress_compact_hip(std::span<const std::uint32_t> permutation,
+                                            FormatOptions format) {
+    return compress_hip(permutation, format);
+}
+
+std::vector<std::byte> compress_compact_hip(std::span<const std::uint64_t> permutation,
+                                            FormatOptions format) {
+    return compress_hip(permutation, format);
+}
+
+double last_device_seconds() noexcept { return benes_hip_last_device_seconds(); }
+
+}  // namespace detail
+}  // namespace benes

@@ -1,0 +1,155 @@
+/* SPDX-License-Identifier: Apache-2.0
+ *
+ * benes_hip.h -- C ABI of the MI355X (gfx950) compression backend for the
+ * compact version-1 Benes permutation format.
+ *
+ * This is the drop-in boundary for the reference's backend plug-in point:
+ *
+ *   bool detail::cuda_available() noexcept;                    src/internal.hpp:1191
+ *   std::vector<std::byte> detail::compress_compact_cuda(
+ *       std::span<const std::uint32_t>, FormatOptions);        src/internal.hpp:1193-1195
+ *   std::vector<std::byte> detail::compress_compact_cuda(
+ *       std::span<const std::uint64_t>, FormatOptions);        src/internal.hpp:1196-1198
+ *
+ * (the same slot the Metal backend fills at src/internal.hpp:1183-1190 and the
+ * stubs at src/backends_stub.cpp:8-40 fill when a backend is not built).
+ * Plain pointers and sizes only; no C++ or torch types.  Status codes map onto
+ * the exception classes the reference throws for the same condition; the
+ * C++ layer (benes-perm_amd/include/benes/benes.hpp) rethrows them.
+ *
+ * Every output blob is byte-identical to the reference's single-threaded
+ * portable encoder (src/benes.cpp:189-382) for both input widths.
+ */
+#ifndef BENES_HIP_H
+#define BENES_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define BENES_HIP_OK 0
+#define BENES_HIP_INVALID_ARGUMENT 1 /* std::invalid_argument */
+#define BENES_HIP_OVERFLOW 2         /* std::overflow_error   */
+#define BENES_HIP_LENGTH 3           /* std::length_error     */
+#define BENES_HIP_LOGIC 4            /* std::logic_error      */
+#define BENES_HIP_RUNTIME 5          /* std::runtime_error (device error, unavailable) */
+#define BENES_HIP_BUFFER_TOO_SMALL 6 /* out_capacity < needed; *out_size holds the need */
+
+/* FormatOptions (reference include/benes/benes.hpp:32-39).  Empty cluster
+ * lists (count 0) select the balanced default, as in the reference. */
+typedef struct benes_hip_format {
+    uint8_t middle_group_log2;     /* 1..5, default 5 */
+    uint8_t middle_placement;      /* 0 separate, 1 embedded_input, 2 embedded_output */
+    uint8_t n_input_clusters;
+    uint8_t n_output_clusters;
+    const uint8_t* input_clusters;
+    const uint8_t* output_clusters;
+} benes_hip_format;
+
+/* Replaces detail::cuda_available() (src/internal.hpp:1191): non-zero when
+ * a gfx950 device is present and the kernels in this library load on it. */
+int benes_hip_available(void);
+
+/* Replaces detail::compress_compact_cuda(span, FormatOptions)
+ * (src/internal.hpp:1193-1198).  `perm` is a host array of n entries of
+ * width_bytes (4 or 8).  Writes the whole blob (header included) to `out`.
+ * When out_capacity is too small, returns BENES_HIP_BUFFER_TOO_SMALL with the
+ * needed size in *out_size (call with out == NULL to query).
+ * *device_seconds (optional) receives the device time of the call, the
+ * reference's last_device_seconds() (src/benes.cpp:119-125). */
+int benes_hip_compress(const void* perm, uint64_t n, uint32_t width_bytes,
+                       const benes_hip_format* format,
+                       uint8_t* out, uint64_t out_capacity, uint64_t* out_size,
+                       double* device_seconds, char* err, size_t err_len);
+
+/* Device-resident variant of benes_hip_compress: `d_perm` and `d_out` are
+ * device pointers on the current device; work is queued on `stream`
+ * (a hipStream_t, NULL for the default stream) and the call returns once the
+ * blob is complete in d_out.  Same validation and error behaviour. */
+int benes_hip_compress_device(const void* d_perm, uint64_t n, uint32_t width_bytes,
+                              const benes_hip_format* format,
+                              uint8_t* d_out, uint64_t out_capacity, uint64_t* out_size,
+                              void* stream, char* err, size_t err_len);
+
+/* Size of the blob for n entries (reference make_layout, src/internal.hpp:389-429). */
+int benes_hip_encoded_size(uint64_t n, const benes_hip_format* format,
+                           uint64_t* out_size, char* err, size_t err_len);
+
+/* Device seconds of the last compress on this thread
+ * (reference detail::last_device_seconds, src/benes.cpp:123-125). */
+double benes_hip_last_device_seconds(void);
+
+/* Per-kernel device timing (no counterpart in the reference, whose only timer
+ * is last_device_seconds): while enabled, every kernel launch of later
+ * compress calls is bracketed by a HIP event pair on the stream it runs on,
+ * and the elapsed times are summed per kernel name.  Enabling clears the sums.
+ * benes_hip_kernel_stats returns 0 once `index` is past the last kernel. */
+void benes_hip_set_profiling(int enable);
+int benes_hip_kernel_stats(int index, char* name, size_t name_len, double* total_ms,
+                           uint64_t* launches);
+
+/* ---- Host-side format arithmetic (no device needed), exported so that the
+ * ---- layout and addressing can be checked against the oracle on a CPU. ---- */
+
+typedef struct benes_hip_layout {
+    uint64_t size;
+    uint64_t network_size;
+    uint64_t input_offset, output_offset, middle_offset, end_offset; /* bytes */
+    uint64_t switch_bits, middle_bits;
+    uint32_t routing_depth;
+    uint8_t middle_group_log2, middle_placement, n_input_clusters, n_output_clusters;
+    uint8_t input_clusters[64];
+    uint8_t output_clusters[64];
+} benes_hip_layout;
+
+/* make_layout + normalize_format (src/internal.hpp:357-429). */
+int benes_hip_layout_of(uint64_t n, const benes_hip_format* format,
+                        benes_hip_layout* out, char* err, size_t err_len);
+
+/* Switch-bit address of node (depth, path) on side 0 (input) or 1 (output):
+ * the reference's SwitchBitAddress (src/internal.hpp:456-473) reached by
+ * SectionAddress::child() along `path` (first branch = most significant of
+ * `depth` bits).  Also the node's count and, when the node is a leaf, the bit
+ * offset of its rank within its section (rank_bit). */
+typedef struct benes_hip_address {
+    uint64_t count;
+    uint64_t packet_base;
+    uint64_t full_packet_bits;
+    uint64_t full_packet_prefix;
+    uint64_t tail_packet_prefix;
+    uint64_t pair_span;
+    uint64_t tail_packet;
+    uint64_t rank_bit;
+    uint64_t offset; /* position of the node's entries in the level array */
+} benes_hip_address;
+
+int benes_hip_node_address(uint64_t n, const benes_hip_format* format, int side,
+                           uint32_t depth, uint64_t path, benes_hip_address* out,
+                           char* err, size_t err_len);
+
+/* Bit offset of switch `pair` of a node (SwitchBitAddress::bit). */
+uint64_t benes_hip_address_bit(const benes_hip_address* address, uint64_t pair);
+
+/* ---- Synthetic seeded permutations (benchmark and test workloads). ---- */
+
+#define BENES_HIP_PERM_FISHER_YATES 0 /* uniform: for i = n-1..1 swap(i, U[0,i]) */
+#define BENES_HIP_PERM_SINGLE_CYCLE 1 /* Sattolo: for i = n-1..1 swap(i, U[0,i-1]) */
+#define BENES_HIP_PERM_NEAR_IDENTITY 2 /* identity + floor(rate_ppm*n/1e6) swaps */
+#define BENES_HIP_PERM_IDENTITY 3
+#define BENES_HIP_PERM_REVERSE 4
+
+/* Writes n entries of width_bytes (4 or 8) into `out` (host memory).  The
+ * generator is splitmix64 with Lemire's bounded draw, so the sequence is fixed
+ * by (kind, n, seed, rate_ppm) on every platform. */
+int benes_hip_generate(uint32_t kind, uint64_t n, uint64_t seed, uint32_t rate_ppm,
+                       void* out, uint32_t width_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BENES_HIP_H */

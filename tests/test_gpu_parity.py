@@ -1,0 +1,175 @@
+# SPDX-License-Identifier: Apache-2.0
+"""Byte parity of the gfx950 backend with the oracle (a restatement of the
+reference's single-threaded portable encoder) and with digests of blobs the
+reference library itself produced (tests/golden/fixtures.json).  Every test
+calls through the C ABI (libbenes_hip.so); nothing falls back to the CPU.
+
+Cases follow the reference's own suite (tests/test_benes.cpp): golden blobs,
+exhaustive small sizes, boundary sizes, every format option and placement,
+uint32 == uint64 bytes, thread/launch determinism, and invalid input.
+"""
+from __future__ import annotations
+
+import hashlib
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+import support as S
+
+B = S.benes_hip
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    # On a GPU run the backend must be there: fail, never skip.
+    assert B.available(), "no gfx950 device or libbenes_hip.so failed to load its kernels"
+
+
+def read_golden(name):
+    with open(os.path.join(S.GOLDEN, name), "rb") as f:
+        return f.read()
+
+
+def test_golden_blobs():
+    assert B.compress(np.arange(32, dtype=np.uint32)) == read_golden("identity-32.benes")
+    assert B.compress(np.arange(33, dtype=np.uint32)[::-1].copy()) == read_golden("reverse-33.benes")
+    blob = read_golden("random-47.benes")
+    perm = S.oracle_decode(blob).astype(np.uint32)
+    assert B.compress(perm) == blob
+    assert B.compress(perm.astype(np.uint64)) == blob
+
+
+@pytest.mark.parametrize("n", range(1, 7))
+def test_exhaustive_small(n):
+    for p in itertools.permutations(range(n)):
+        perm = np.array(p, np.uint32)
+        assert B.compress(perm) == S.oracle_compress(perm), p
+
+
+BOUNDARY = [9, 15, 17, 30, 31, 33, 34, 47, 62, 63, 65, 66, 95, 96, 97, 126, 127, 129, 130, 255,
+            257, 511, 513, 1023, 1025, 4095, 4097]
+
+
+@pytest.mark.parametrize("n", BOUNDARY)
+def test_boundaries(n):
+    rng = np.random.default_rng(0xC04FAC70 + n)
+    for kind in range(3):
+        perm = {0: rng.permutation(n), 1: np.arange(n), 2: np.arange(n)[::-1]}[kind]
+        perm = np.ascontiguousarray(perm, dtype=np.uint32)
+        assert B.compress(perm) == S.oracle_compress(perm), (n, kind)
+
+
+@pytest.mark.parametrize("n", [3, 7, 15, 17, 31, 33, 63, 65, 127, 129, 257, 1000, 4097])
+def test_format_options(n):
+    rng = np.random.default_rng(0xF04A7000 + n)
+    perm = rng.permutation(n).astype(np.uint32)
+    for g in (1, 2, 3, 4, 5):
+        d = S.routing_depth(n, g)
+        splits = [()] + ([tuple([1] * d)] if d else []) + ([(d - 1, 1), (1, d - 1), (d,)] if d >= 2 else [])
+        sizes = set()
+        for pl in (0, 1, 2):
+            for ic in splits:
+                oc = splits[(len(ic) + pl) % len(splits)]
+                got = B.compress(perm, S.fmt(g, pl, ic, oc))
+                assert got == S.oracle_compress(perm, g, pl, ic, oc), (n, g, pl, ic, oc)
+                if ic == () and oc == ():
+                    sizes.add(len(got))
+        assert len(sizes) == 1
+
+
+def test_reference_format_case_4097():
+    # tests/test_benes.cpp:318-343: g = 4, input {3, 6}, output {5, 4}, embedded_input
+    perm = np.random.default_rng(0xF04A7FFF).permutation(4097).astype(np.uint32)
+    args = (4, 1, (3, 6), (5, 4))
+    assert B.compress(perm, S.fmt(*args)) == S.oracle_compress(perm, *args)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 31, 32, 33, 63, 65, 127, 129, 4097, 70001])
+def test_uint64_same_bytes(n):
+    # tests/test_benes.cpp:230-266
+    perm = np.random.default_rng(0x64000000 + n).permutation(n).astype(np.uint32)
+    d = S.routing_depth(n, 4)
+    args = (4, 1 if n % 2 == 0 else 2, tuple([1] * d), (d,) if d else ())
+    ref = S.oracle_compress(perm, *args)
+    assert B.compress(perm, S.fmt(*args)) == ref
+    assert B.compress(perm.astype(np.uint64), S.fmt(*args)) == ref
+
+
+@pytest.mark.parametrize("n", [16383, 16384, 16385, 32767, 32769, 65537, 100003, 262145])
+def test_global_levels(n):
+    # Sizes above one workgroup's LDS subtree exercise the HBM union-find,
+    # resolve and packet-packing kernels, including odd nodes and tail packets.
+    rng = np.random.default_rng(n)
+    for g, pl in ((5, 0), (3, 1), (4, 2)):
+        d = S.routing_depth(n, g)
+        for ic, oc in (((), ()), (tuple([1] * d), (d,)), ((2, d - 3, 1), (d - 1, 1))):
+            perm = rng.permutation(n).astype(np.uint32)
+            assert B.compress(perm, S.fmt(g, pl, ic, oc)) == S.oracle_compress(perm, g, pl, ic, oc)
+
+
+def test_structured_inputs_large():
+    n = 300007
+    for kind in (B.SINGLE_CYCLE, B.NEAR_IDENTITY, B.IDENTITY, B.REVERSE):
+        perm = B.generate(kind, n, 77, 5000)
+        assert B.compress(perm) == S.oracle_compress(perm), kind
+
+
+def load_fixtures():
+    with open(os.path.join(S.GOLDEN, "fixtures.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("rec", load_fixtures(), ids=lambda r: r["name"])
+def test_reference_digests(rec):
+    perm = B.generate(rec["kind"], rec["n"], rec["seed"], rec["rate"], rec["width"])
+    blob = B.compress(perm, S.fmt(rec["g"], rec["placement"], rec["ic"], rec["oc"]))
+    assert len(blob) == rec["size"]
+    assert hashlib.sha256(blob).hexdigest() == rec["sha256"], rec["name"]
+
+
+def test_repeatable():
+    perm = B.generate(B.FISHER_YATES, 1 << 20, 1)
+    first = B.compress(perm)
+    for _ in range(3):
+        assert B.compress(perm) == first
+
+
+def test_invalid_input():
+    with pytest.raises(ValueError):
+        B.compress(np.zeros(0, np.uint32))
+    with pytest.raises(ValueError):
+        B.compress(np.array([0, 1, 1], np.uint32))
+    with pytest.raises(ValueError):
+        B.compress(np.array([0, 1, 3], np.uint32))
+    with pytest.raises(ValueError):
+        B.compress(np.array([0, 1, 1], np.uint64))
+    big = np.arange(100000, dtype=np.uint32)
+    big[99999] = 5
+    with pytest.raises(ValueError):
+        B.compress(big)
+    perm = np.random.default_rng(1).permutation(4097).astype(np.uint32)
+    with pytest.raises(ValueError):
+        B.compress(perm, S.fmt(ic=(1, 1)))
+    with pytest.raises(ValueError):
+        B.compress(perm, S.fmt(g=6))
+    # the device is still healthy afterwards
+    assert B.compress(perm) == S.oracle_compress(perm)
+
+
+def test_device_entry_point():
+    torch = pytest.importorskip("torch")
+    n = (1 << 20) + 3
+    perm = B.generate(B.FISHER_YATES, n, 5)
+    fmt = S.fmt(5, 2)
+    host = B.compress(perm, fmt)
+    d_perm = torch.from_numpy(perm.view(np.int32)).cuda()
+    d_out = torch.zeros(len(host), dtype=torch.uint8, device="cuda")
+    size = B.compress_device(d_perm.data_ptr(), n, 4, d_out.data_ptr(), d_out.numel(), fmt,
+                             stream=torch.cuda.current_stream().cuda_stream)
+    assert size == len(host)
+    assert d_out.cpu().numpy().tobytes() == host
