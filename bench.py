@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "benes-perm_amd"), os.path.join(ROOT, "tests")]
 
 import numpy as np  # noqa: E402
+import torch  # noqa: E402  (first: libbenes_hip.so then shares torch's HIP runtime)
 
 import benes_hip as B  # noqa: E402
 
@@ -103,8 +104,7 @@ def main():
               "batch) per device and has no multi-GPU path; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
+    torch.cuda.init()
     assert B.available(), "gfx950 backend unavailable"
     t0 = time.perf_counter()
     perm = B.generate(B.FISHER_YATES, N, SEED)

@@ -274,6 +274,67 @@ __device__ __forceinline__ void rank_leaf(const Plan& P, uint32_t d, uint64_t pa
     if (x2) atomicOr(b + w + 2, static_cast<unsigned long long>(x2));
 }
 
+// Writes the switch bits of every internal node at relative level k of a
+// subtree (absolute depth d0 + k) into the blob.  Bits are produced packet by
+// packet: within a cluster group, one packet's share of a level is the
+// concatenation of that packet's pair range from each node in path order
+// (internal.hpp:604-649), so a run of consecutive nodes is one contiguous
+// stretch of the blob, written as whole words.  `sbits`/`tbits` hold the
+// level's switch values at the position of each node's pair in LDS.
+__device__ __forceinline__ void emit_level_runs(const Plan& P, uint32_t d0, uint64_t b0,
+                                                uint64_t n0, uint32_t k,
+                                                const uint64_t* sbits, const uint64_t* tbits,
+                                                const uint16_t* noff, uint64_t* blob) {
+    const uint32_t d = d0 + k;
+    const uint32_t L = P.leaf;
+    const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const uint64_t cmax = (n0 + ((uint64_t{1} << k) - 1)) >> k;
+    const uint64_t maxpairs = cmax >> 1;
+    for (int side = 0; side < 2; ++side) {
+        const SideDepth& sd = P.side[side][d];
+        const uint32_t r = sd.relative_level, g = sd.group_depth;
+        const uint32_t R = d - r;
+        const uint32_t span_log2 = g - r - 1;
+        const uint64_t span = uint64_t{1} << span_log2;
+        const uint64_t* rows = side == 0 ? sbits : tbits;
+        const uint64_t secbase = (side == 0 ? P.input_offset : P.output_offset) * 8;
+        // Sets of consecutive nodes that share one cluster group.
+        const bool one_set = R <= d0;
+        const uint64_t nsets = one_set ? 1 : (uint64_t{1} << (R - d0));
+        const uint64_t set_nodes = one_set ? (uint64_t{1} << k) : (uint64_t{1} << r);
+        const uint64_t packets = (maxpairs + span - 1) >> span_log2;
+        for (uint64_t job = wave; job < nsets * packets; job += nwaves) {
+            const uint64_t set = job / packets, pk = job % packets;
+            const uint64_t first = set * set_nodes;
+            const uint64_t minc = one_set ? (n0 >> k) : (node_count(n0, R - d0, set) >> r);
+            const uint64_t first_pair = pk << span_log2;
+            if (minc > L && (minc >> 1) >= first_pair + span) {
+                // Every node fills its share of this packet: one run.
+                const SideAddr a = node_address(P, side, d, (b0 << k) | first);
+                wave_write_run(blob, secbase + bit_of(a, first_pair), set_nodes << span_log2,
+                               [&](uint64_t q) {
+                                   const uint64_t pos = noff[first + (q >> span_log2)] +
+                                                        first_pair + (q & (span - 1));
+                                   return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
+                               });
+            } else {
+                for (uint64_t pi = first; pi < first + set_nodes; ++pi) {
+                    const uint64_t c = node_count(n0, k, pi);
+                    if (c <= L) continue;
+                    const uint64_t len = packet_share(c, pk, span_log2, L);
+                    if (len == 0) continue;
+                    const SideAddr a = node_address(P, side, d, (b0 << k) | pi);
+                    const uint64_t base = noff[pi] + first_pair;
+                    wave_write_run(blob, secbase + bit_of(a, first_pair), len, [&](uint64_t q) {
+                        const uint64_t pos = base + q;
+                        return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
+                    });
+                }
+            }
+        }
+    }
+}
+
 template <typename V>
 __global__ __launch_bounds__(kSubThreads) void k_subtree(const V* __restrict__ src,
                                                          const Plan* __restrict__ plan,
@@ -284,6 +345,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const V* __restrict__ s
     __shared__ uint32_t uf[kSub];
     __shared__ uint64_t sbits[kSub / 64];
     __shared__ uint64_t tbits[kSub / 64];
+    __shared__ uint16_t noff[kSub / 2];
     const Plan& P = *plan;
     const uint32_t L = P.leaf;
     const uint64_t b0 = blockIdx.x;
@@ -347,24 +409,11 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const V* __restrict__ s
                 tbits[wbase >> 6] = mt;
             }
         }
-        __syncthreads();
-        for (uint64_t pi = wave; pi < nodes; pi += nwaves) {
-            const uint64_t c = node_count(n0, k, pi);
-            if (c <= L) continue;
-            const uint64_t off = node_offset(n0, k, pi), h = c >> 1;
-            const uint64_t path = (b0 << k) | pi;
-            for (int side = 0; side < 2; ++side) {
-                const SideAddr a = node_address(P, side, d, path);
-                const uint64_t secbase = (side == 0 ? P.input_offset : P.output_offset) * 8;
-                const uint64_t* rows = side == 0 ? sbits : tbits;
-                for (uint64_t c0 = 0; c0 < h; c0 += 64) {
-                    const uint64_t p = c0 + lane;
-                    const bool valid = p < h;
-                    const bool bit = valid && ((rows[(off + p) >> 6] >> ((off + p) & 63)) & 1);
-                    wave_or_bits(blob, secbase + (valid ? bit_of(a, p) : 0), bit, valid);
-                }
-            }
+        for (uint64_t pi = tid; pi < nodes; pi += nthreads) {
+            noff[pi] = static_cast<uint16_t>(node_offset(n0, k, pi));
         }
+        __syncthreads();
+        emit_level_runs(P, d0, b0, n0, k, sbits, tbits, noff, blob);
         __syncthreads();
         cur ^= 1;
     }
