@@ -79,7 +79,7 @@ struct Workspace {
     int device = -1;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-    DevBuf input, level[2], uf, srows, trows, seen, desc, plan, err, blob;
+    DevBuf input, level[2], ilevel[2], inv0, uf, srows, trows, desc, plan, err, blob;
     void init(int dev) {
         if (stream) return;
         device = dev;
@@ -162,26 +162,34 @@ void enqueue_pipeline(Workspace& ws, const V* src, const Layout& lay, Profiler& 
     ws.blob.ensure(blob_words * 8);
     ws.plan.ensure(sizeof(Plan));
     ws.err.ensure(sizeof(int));
-    ws.seen.ensure(ceil_div(n, 32) * 4);
     BH_CHECK(hipMemcpyAsync(ws.plan.ptr, &plan, sizeof(Plan), hipMemcpyHostToDevice, st));
     BH_CHECK(hipMemsetAsync(ws.err.ptr, 0, sizeof(int), st));
-    BH_CHECK(hipMemsetAsync(ws.seen.ptr, 0, ceil_div(n, 32) * 4, st));
     BH_CHECK(hipMemsetAsync(ws.blob.ptr, 0, blob_words * 8, st));
-    const int* err = ws.err.as<int>();
+    int* err = ws.err.as<int>();
     const Plan* dplan = ws.plan.as<Plan>();
     uint64_t* blob = ws.blob.as<uint64_t>();
-    {
-        const uint64_t blocks = std::min<uint64_t>(ceil_div(n, kBlock), 65536);
-        BH_LAUNCH(prof, "k_validate", k_validate<V>, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, st,
-                           src, n, ws.seen.as<uint32_t>(), ws.err.as<int>());
-    }
     const uint32_t df = std::min<uint32_t>(fusion_depth(n), plan.depth);
+    // Validation and the inverse: 32-bit while every index fits below the
+    // all-ones sentinel.
+    const bool wide_inverse = n >= 0xFFFFFFFFull;
+    const uint32_t blocks_n = static_cast<uint32_t>(ceil_div(n, kBlock));
+    ws.inv0.ensure(n * (wide_inverse ? 8 : 4));
+    BH_CHECK(hipMemsetAsync(ws.inv0.ptr, 0xFF, n * (wide_inverse ? 8 : 4), st));
+    auto validate = [&](auto* inv) {
+        using VI = std::remove_pointer_t<decltype(inv)>;
+        BH_LAUNCH(prof, "k_scatter_inverse", (k_scatter_inverse<V, VI>), dim3(blocks_n),
+                  dim3(kBlock), 0, st, src, n, inv, err);
+        BH_LAUNCH(prof, "k_check_inverse", (k_check_inverse<VI>), dim3(blocks_n), dim3(kBlock), 0,
+                  st, inv, n, err);
+    };
+    if (wide_inverse) validate(ws.inv0.as<uint64_t>());
+    else validate(ws.inv0.as<uint32_t>());
     if (df > 0) {
-        ws.level[0].ensure(n * 4);
-        ws.level[1].ensure(n * 4);
+        for (int i = 0; i < 2; ++i) {
+            ws.level[i].ensure(n * 4);
+            ws.ilevel[i].ensure(n * 4);
+        }
     }
-    const void* cur = src;
-    bool cur_is_input = true;
     int next = 0;
     for (uint32_t d = 0; d < df; ++d) {
         const uint64_t nodes = uint64_t{1} << d;
@@ -192,29 +200,42 @@ void enqueue_pipeline(Workspace& ws, const V* src, const Layout& lay, Profiler& 
         ws.srows.ensure(nodes * row_words * 8);
         ws.trows.ensure(nodes * row_words * 8);
         NodeDesc* desc = ws.desc.as<NodeDesc>();
-        BH_LAUNCH(prof, "k_describe", k_describe, dim3(static_cast<uint32_t>(ceil_div(nodes, 64))), dim3(64),
-                           0, st, dplan, d, row_words, desc, err);
-        const bool wide = cmax >= (uint64_t{1} << 31);
+        BH_LAUNCH(prof, "k_describe", k_describe, dim3(static_cast<uint32_t>(ceil_div(nodes, 64))),
+                  dim3(64), 0, st, dplan, d, row_words, desc, err);
+        // Input slots of a node number ceil(c/2) < 2^31 for 32-bit words.
+        const bool wide = cmax / 2 + 1 >= (uint64_t{1} << 31);
         ws.uf.ensure(n * (wide ? 8 : 4));
         BH_CHECK(hipMemsetAsync(ws.uf.ptr, 0xFF, n * (wide ? 8 : 4), st));
-        const uint64_t uchunks = ceil_div(cmax, kBlock);
-        const uint64_t rchunks = ceil_div(maxpairs + 1, kBlock);
-        uint32_t* dst = ws.level[next].as<uint32_t>();
-        auto run = [&](auto src_typed, auto uf_typed) {
-            using VS = std::remove_const_t<std::remove_pointer_t<decltype(src_typed)>>;
-            using W = std::remove_pointer_t<decltype(uf_typed)>;
-            BH_LAUNCH(prof, "k_union", (k_union<VS, W>), dim3(static_cast<uint32_t>(uchunks * nodes)),
-                               dim3(kBlock), 0, st, src_typed, uf_typed, desc, uchunks, err);
-            BH_LAUNCH(prof, "k_resolve", (k_resolve<VS, W>), dim3(static_cast<uint32_t>(rchunks * nodes)),
-                               dim3(kBlock), 0, st, src_typed, dst, uf_typed, desc, rchunks,
-                               ws.srows.as<uint64_t>(), ws.trows.as<uint64_t>(), err);
+        const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
+        const dim3 grid(static_cast<uint32_t>(chunks * nodes));
+        uint32_t* pdst = ws.level[next].as<uint32_t>();
+        uint32_t* idst = ws.ilevel[next].as<uint32_t>();
+        uint64_t* srows = ws.srows.as<uint64_t>();
+        uint64_t* trows = ws.trows.as<uint64_t>();
+        auto run = [&](auto psrc, auto isrc, auto uf) {
+            using VS = std::remove_const_t<std::remove_pointer_t<decltype(psrc)>>;
+            using VI = std::remove_const_t<std::remove_pointer_t<decltype(isrc)>>;
+            using W = std::remove_pointer_t<decltype(uf)>;
+            BH_LAUNCH(prof, "k_union", (k_union<VI, W>), grid, dim3(kBlock), 0, st, isrc, uf, desc,
+                      nodes, chunks, err);
+            BH_LAUNCH(prof, "k_route_inputs", (k_route_inputs<VS, W>), grid, dim3(kBlock), 0, st,
+                      psrc, pdst, uf, desc, nodes, chunks, srows, err);
+            BH_LAUNCH(prof, "k_route_outputs", (k_route_outputs<VI>), grid, dim3(kBlock), 0, st,
+                      isrc, idst, desc, nodes, chunks, srows, trows, err);
         };
-        if (cur_is_input) {
-            if (wide) run(static_cast<const V*>(cur), ws.uf.as<uint64_t>());
-            else run(static_cast<const V*>(cur), ws.uf.as<uint32_t>());
+        if (d == 0) {
+            if (wide_inverse) {
+                if (wide) run(src, ws.inv0.as<uint64_t>(), ws.uf.as<uint64_t>());
+                else run(src, ws.inv0.as<uint64_t>(), ws.uf.as<uint32_t>());
+            } else {
+                if (wide) run(src, ws.inv0.as<uint32_t>(), ws.uf.as<uint64_t>());
+                else run(src, ws.inv0.as<uint32_t>(), ws.uf.as<uint32_t>());
+            }
         } else {
-            if (wide) run(static_cast<const uint32_t*>(cur), ws.uf.as<uint64_t>());
-            else run(static_cast<const uint32_t*>(cur), ws.uf.as<uint32_t>());
+            const uint32_t* ps = ws.level[next ^ 1].as<uint32_t>();
+            const uint32_t* is = ws.ilevel[next ^ 1].as<uint32_t>();
+            if (wide) run(ps, is, ws.uf.as<uint64_t>());
+            else run(ps, is, ws.uf.as<uint32_t>());
         }
         for (int side = 0; side < 2; ++side) {
             const SideDepth& sd = plan.side[side][d];
@@ -222,23 +243,20 @@ void enqueue_pipeline(Workspace& ws, const V* src, const Layout& lay, Profiler& 
             const uint64_t packets = ceil_div(maxpairs, uint64_t{1} << span_log2);
             const uint64_t groups = uint64_t{1} << (d - sd.relative_level);
             const uint64_t waves = groups * packets;
-            BH_LAUNCH(prof, "k_pack", k_pack, dim3(static_cast<uint32_t>(ceil_div(waves, kBlock / 64))),
-                               dim3(kBlock), 0, st, dplan, desc, side, d, packets,
-                               side == 0 ? ws.srows.as<uint64_t>() : ws.trows.as<uint64_t>(),
-                               blob, err);
+            BH_LAUNCH(prof, "k_pack", k_pack,
+                      dim3(static_cast<uint32_t>(ceil_div(waves, kBlock / 64))), dim3(kBlock), 0,
+                      st, dplan, desc, side, d, packets, side == 0 ? srows : trows, blob, err);
         }
-        cur = dst;
-        cur_is_input = false;
         next ^= 1;
     }
     const uint64_t subtrees = uint64_t{1} << df;
-    if (cur_is_input) {
-        BH_LAUNCH(prof, "k_subtree", k_subtree<V>, dim3(static_cast<uint32_t>(subtrees)), dim3(kSubThreads),
-                           0, st, static_cast<const V*>(cur), dplan, df, blob, err);
+    if (df == 0) {
+        BH_LAUNCH(prof, "k_subtree", k_subtree<V>, dim3(static_cast<uint32_t>(subtrees)),
+                  dim3(kSubThreads), 0, st, src, dplan, df, blob, err);
     } else {
         BH_LAUNCH(prof, "k_subtree", k_subtree<uint32_t>, dim3(static_cast<uint32_t>(subtrees)),
-                           dim3(kSubThreads), 0, st, static_cast<const uint32_t*>(cur), dplan, df,
-                           blob, err);
+                  dim3(kSubThreads), 0, st, ws.level[next ^ 1].as<uint32_t>(), dplan, df, blob,
+                  err);
     }
     BH_CHECK(hipGetLastError());
 }

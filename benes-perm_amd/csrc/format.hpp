@@ -145,7 +145,8 @@ BH_HD uint64_t level_prefix_bits(uint64_t C, uint32_t g, uint32_t l,
     const uint64_t small = C >> l;
     const uint64_t m = C & ((uint64_t{1} << l) - 1);
     const uint32_t span_log2 = g - l - 1;
-    const uint64_t inc = large_before(k, m, l);
+    // All m large nodes lie among all 2^l paths.
+    const uint64_t inc = k == (uint64_t{1} << l) ? m : large_before(k, m, l);
     return inc * packet_share(small + 1, packet, span_log2, leaf) +
            (k - inc) * packet_share(small, packet, span_log2, leaf);
 }
@@ -173,10 +174,8 @@ struct Walk {
     uint32_t pad;
 };
 
-// Walks from the root to node (d, path), accumulating what the reference's
-// SectionAddress::child() accumulates (internal.hpp:553-601): the residual
-// bits of every left sibling passed on the way, and each new group's base.
-BH_HD Walk walk_to(const Plan& p, int side, uint32_t d, uint64_t path) {
+// State of a walk standing at the root.
+BH_HD Walk walk_root(const Plan& p, int side) {
     Walk w{};
     w.root_count = p.n;
     const SideDepth* sd = p.side[side];
@@ -185,7 +184,16 @@ BH_HD Walk walk_to(const Plan& p, int side, uint32_t d, uint64_t path) {
         w.full_packet_bits = sd[0].full_packet_bits[0];
         w.group_depth = sd[0].group_depth;
     }
-    for (uint32_t i = 1; i <= d; ++i) {
+    return w;
+}
+
+// Continues a walk standing at depth `from` on the way to node (d, path),
+// accumulating what the reference's SectionAddress::child() accumulates
+// (internal.hpp:553-601): the residual bits of every left sibling passed on
+// the way, and each new group's base.
+BH_HD Walk walk_from(const Plan& p, int side, Walk w, uint32_t from, uint32_t d, uint64_t path) {
+    const SideDepth* sd = p.side[side];
+    for (uint32_t i = from + 1; i <= d; ++i) {
         const uint64_t here = path >> (d - i);
         if (here & 1) {
             const uint64_t sibling = here - 1;
@@ -213,9 +221,13 @@ BH_HD Walk walk_to(const Plan& p, int side, uint32_t d, uint64_t path) {
     return w;
 }
 
-// SwitchBitAddress of internal node (d, path) on one side.
-BH_HD SideAddr node_address(const Plan& p, int side, uint32_t d, uint64_t path) {
-    const Walk w = walk_to(p, side, d, path);
+BH_HD Walk walk_to(const Plan& p, int side, uint32_t d, uint64_t path) {
+    return walk_from(p, side, walk_root(p, side), 0, d, path);
+}
+
+// SwitchBitAddress of internal node (d, path) on one side, from the walk
+// that reached it.
+BH_HD SideAddr address_at(const Plan& p, const Walk& w, uint32_t d, uint64_t path) {
     SideAddr a{};
     const uint32_t g = w.group_depth, r = w.relative_level;
     const uint64_t local = path & ((uint64_t{1} << r) - 1);
@@ -228,6 +240,11 @@ BH_HD SideAddr node_address(const Plan& p, int side, uint32_t d, uint64_t path) 
     a.full_prefix = packet_prefix(w.root_count, g, r, local, 0, p.leaf);
     a.tail_prefix = packet_prefix(w.root_count, g, r, local, a.tail_packet, p.leaf);
     return a;
+}
+
+// SwitchBitAddress of internal node (d, path) on one side.
+BH_HD SideAddr node_address(const Plan& p, int side, uint32_t d, uint64_t path) {
+    return address_at(p, walk_to(p, side, d, path), d, path);
 }
 
 // Bit offset of the factoradic rank of leaf (d, path) within its section:
@@ -244,6 +261,12 @@ BH_HD uint64_t leaf_rank_bit(const Plan& p, uint32_t d, uint64_t path) {
     }
     const Walk w = walk_to(p, p.placement == 1 ? 0 : 1, d, path);
     if (w.root_count <= p.leaf) return w.packet_base;
+    return w.packet_base + w.group_switch_bits + w.preceding;
+}
+
+// Embedded rank offset from a walk that reached the leaf.
+BH_HD uint64_t rank_bit_at(const Walk& w, uint32_t leaf) {
+    if (w.root_count <= leaf) return w.packet_base;
     return w.packet_base + w.group_switch_bits + w.preceding;
 }
 

@@ -7,13 +7,18 @@
 // Levels ping-pong between two u32 arrays, so routing is a pure streaming
 // pass; only the union-find touches memory at random.
 //
+// Before the first level:
+//   k_scatter_inverse  validation and the inverse permutation, no atomics
 // Per "global" level (nodes larger than one workgroup's LDS):
-//   k_describe  one thread per node: offsets and both sides' SwitchBitAddress
-//   k_union     one thread per entry: parity union-find in HBM
-//   k_resolve   one thread per pair: canonical switch values, child routing,
-//               switch values ballot-packed into node-major bit rows
-//   k_pack      one wave per (cluster group, packet): gathers a packet's run
-//               of switch bits from the rows and writes whole 64-bit words
+//   k_describe       one thread per node: offsets and both sides' addresses
+//   k_union          one thread per output pair: parity union-find over the
+//                    node's input slots in HBM (one CAS per hook)
+//   k_route_inputs   one thread per input pair: switch values, routing of P,
+//                    values ballot-packed into node-major bit rows
+//   k_route_outputs  one thread per output pair: output switch values from
+//                    the input rows, routing of the inverse
+//   k_pack           one wave per (cluster group, packet): gathers a packet's
+//                    run of switch bits from the rows into whole 64-bit words
 // Below the fusion depth:
 //   k_subtree   one workgroup per subtree: every remaining level in LDS, then
 //               the factoradic ranks of its leaves.
@@ -94,26 +99,57 @@ __device__ __forceinline__ uint32_t row_bit(const uint64_t* rows, uint64_t row, 
 }
 
 // ---------------------------------------------------------------------------
-// Input validation: every value in [0, n) exactly once (benes.cpp:152-173).
+// Input validation and the inverse (benes.cpp:152-173).
+//
+// inv starts as all-ones; every in-range entry stores its index at its value.
+// A value out of range raises the error flag; a duplicate leaves some other
+// slot unwritten, which the first union pass (it reads every inverse pair)
+// reports.  No atomics: on this device every global atomic costs ~4x a random
+// store.
 
-template <typename V>
-__global__ void k_validate(const V* __restrict__ perm, uint64_t n, uint32_t* __restrict__ seen,
-                           int* __restrict__ err) {
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += stride) {
-        const uint64_t v = static_cast<uint64_t>(perm[i]);
-        bool bad = v >= n;
-        if (!bad) {
-            const uint32_t bit = 1u << (v & 31);
-            bad = (atomicOr(seen + (v >> 5), bit) & bit) != 0;
-        }
-        if (bad) atomicOr(err, 1);
+template <typename V, typename VI>
+__global__ __launch_bounds__(kBlock) void k_scatter_inverse(const V* __restrict__ perm,
+                                                            uint64_t n, VI* __restrict__ inv,
+                                                            int* __restrict__ err) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t v = static_cast<uint64_t>(perm[i]);
+    if (v >= n) {
+        *err = 1;
+        return;
     }
+    inv[v] = static_cast<VI>(i);
 }
 
 // ---------------------------------------------------------------------------
 // Global levels
+//
+// Each node keeps its permutation P and its inverse I (both node-local).
+// The colouring is a parity union-find over the node's input slots only: the
+// two entries I[2q], I[2q+1] reaching output pair q must take different
+// colours, which links their input slots with label 1 ^ (I[2q] & 1) ^
+// (I[2q+1] & 1).  Output switch values then follow from the input ones:
+// t_q = colour(I[2q]).  Children of both arrays are streaming passes:
+// left P[p] = P[2p + s_p] >> 1 and left I[q] = I[2q + t_q] >> 1.
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// When a level has at least 8 nodes, blocks are renumbered so that every
+// block of node j lands on XCD j % 8: the node's union-find words, arrays and
+// rows then live in that one XCD's L2.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ void block_to_node(uint64_t nodes, uint64_t chunks, uint64_t& node,
+                                              uint64_t& chunk) {
+    const uint64_t b = blockIdx.x;
+    if (nodes % kXcds != 0) {
+        node = b / chunks;
+        chunk = b % chunks;
+        return;
+    }
+    const uint64_t xcd = b % kXcds, local = b / kXcds;
+    node = (local / chunks) * kXcds + xcd;
+    chunk = local % chunks;
+}
 
 __global__ void k_describe(const Plan* __restrict__ plan, uint32_t d, uint64_t row_words,
                            NodeDesc* __restrict__ desc, const int* __restrict__ err) {
@@ -130,38 +166,61 @@ __global__ void k_describe(const Plan* __restrict__ plan, uint32_t d, uint64_t r
     desc[j] = nd;
 }
 
-template <typename V, typename W>
-__global__ __launch_bounds__(kBlock) void k_union(const V* __restrict__ src, W* __restrict__ uf,
-                                                  const NodeDesc* __restrict__ desc,
-                                                  uint64_t chunks, const int* __restrict__ err) {
-    if (*err) return;
-    const uint64_t node = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
-    const uint64_t off = desc[node].off, cnt = desc[node].cnt;
-    const uint64_t x = chunk * kBlock + threadIdx.x;
-    if (x >= cnt) return;
-    const uint64_t v = static_cast<uint64_t>(src[off + x]);
-    if ((cnt & 1) && v == cnt - 1) return;  // edge into the unpaired output
-    uf_union<AgentScope>(uf + off, input_slot<W>(x, cnt), output_pair_slot<W>(v >> 1, cnt),
-                         static_cast<uint32_t>((x ^ v) & 1));
+// Input slot of entry x of a node of `count` entries: the odd tail is slot 0.
+__device__ __forceinline__ uint64_t in_slot(uint64_t x, uint64_t count) {
+    if (count & 1) return x == count - 1 ? 0 : (x >> 1) + 1;
+    return x >> 1;
 }
 
-template <typename V, typename W>
-__global__ __launch_bounds__(kBlock) void k_resolve(const V* __restrict__ src,
-                                                    uint32_t* __restrict__ dst, W* __restrict__ uf,
-                                                    const NodeDesc* __restrict__ desc,
-                                                    uint64_t chunks, uint64_t* __restrict__ srows,
-                                                    uint64_t* __restrict__ trows,
-                                                    const int* __restrict__ err) {
+// A slot of the inverse still all-ones was reached by no entry: some value
+// occurred twice.
+template <typename VI>
+__global__ __launch_bounds__(kBlock) void k_check_inverse(const VI* __restrict__ inv, uint64_t n,
+                                                          int* __restrict__ err) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && inv[i] == static_cast<VI>(~VI{0})) *err = 1;
+}
+
+// One thread per output pair: union of the two input slots that reach it.
+template <typename VI, typename W>
+__global__ __launch_bounds__(kBlock) void k_union(const VI* __restrict__ inv,
+                                                  W* __restrict__ uf,
+                                                  const NodeDesc* __restrict__ desc,
+                                                  uint64_t nodes, uint64_t chunks,
+                                                  const int* __restrict__ err) {
     if (*err) return;
-    const uint64_t node = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    uint64_t node, chunk;
+    block_to_node(nodes, chunks, node, chunk);
+    const uint64_t off = desc[node].off, cnt = desc[node].cnt;
+    const uint64_t q = chunk * kBlock + threadIdx.x;
+    if (q >= (cnt >> 1)) return;
+    const uint64_t a = static_cast<uint64_t>(inv[off + 2 * q]);
+    const uint64_t b = static_cast<uint64_t>(inv[off + 2 * q + 1]);
+    const uint64_t sa = in_slot(a, cnt), sb = in_slot(b, cnt);
+    if (sa == sb) return;  // both entries of one input pair: a 2-cycle
+    uf_union_slots<DeviceForest>(uf + off, static_cast<W>(sa), static_cast<W>(sb),
+                                 static_cast<uint32_t>(1 ^ ((a ^ b) & 1)));
+}
+
+// One thread per input pair: canonical switch value s_p (its slot's parity to
+// the root), child routing of P, and s ballot-packed into the node's row.
+template <typename V, typename W>
+__global__ __launch_bounds__(kBlock) void k_route_inputs(const V* __restrict__ src,
+                                                         uint32_t* __restrict__ dst,
+                                                         W* __restrict__ uf,
+                                                         const NodeDesc* __restrict__ desc,
+                                                         uint64_t nodes, uint64_t chunks,
+                                                         uint64_t* __restrict__ srows,
+                                                         const int* __restrict__ err) {
+    if (*err) return;
+    uint64_t node, chunk;
+    block_to_node(nodes, chunks, node, chunk);
     const uint64_t off = desc[node].off, cnt = desc[node].cnt, row = desc[node].row;
     const uint64_t h = cnt >> 1, lc = h + (cnt & 1);
     const uint64_t p = chunk * kBlock + threadIdx.x;
-    uint32_t s = 0, t = 0;
+    uint32_t s = 0;
     if (p < h) {
-        W* u = uf + off;
-        uf_find<AgentScope>(u, input_pair_slot<W>(p, cnt), s);
-        uf_find<AgentScope>(u, output_pair_slot<W>(p, cnt), t);
+        uf_find<DeviceForest>(uf + off, static_cast<W>(p + (cnt & 1)), s);
         const uint64_t a0 = static_cast<uint64_t>(src[off + 2 * p]);
         const uint64_t a1 = static_cast<uint64_t>(src[off + 2 * p + 1]);
         dst[off + p] = static_cast<uint32_t>((s ? a1 : a0) >> 1);
@@ -169,12 +228,41 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const V* __restrict__ src,
     } else if ((cnt & 1) && p == h) {
         dst[off + h] = static_cast<uint32_t>(static_cast<uint64_t>(src[off + cnt - 1]) >> 1);
     }
-    const uint64_t ms = __ballot(s != 0), mt = __ballot(t != 0);
+    const uint64_t ms = __ballot(s != 0);
     const uint64_t first = p - lane_id();
-    if (lane_id() == 0 && first < h) {
-        srows[row + (first >> 6)] = ms;
-        trows[row + (first >> 6)] = mt;
+    if (lane_id() == 0 && first < h) srows[row + (first >> 6)] = ms;
+}
+
+// One thread per output pair: t_q = colour(I[2q]) read from the input rows,
+// child routing of I, and t ballot-packed into the node's row.
+template <typename VI>
+__global__ __launch_bounds__(kBlock) void k_route_outputs(const VI* __restrict__ inv,
+                                                          uint32_t* __restrict__ dst,
+                                                          const NodeDesc* __restrict__ desc,
+                                                          uint64_t nodes, uint64_t chunks,
+                                                          const uint64_t* __restrict__ srows,
+                                                          uint64_t* __restrict__ trows,
+                                                          const int* __restrict__ err) {
+    if (*err) return;
+    uint64_t node, chunk;
+    block_to_node(nodes, chunks, node, chunk);
+    const uint64_t off = desc[node].off, cnt = desc[node].cnt, row = desc[node].row;
+    const uint64_t h = cnt >> 1, lc = h + (cnt & 1);
+    const uint64_t q = chunk * kBlock + threadIdx.x;
+    uint32_t t = 0;
+    if (q < h) {
+        const uint64_t a0 = static_cast<uint64_t>(inv[off + 2 * q]);
+        const uint64_t a1 = static_cast<uint64_t>(inv[off + 2 * q + 1]);
+        // colour(x) = s_{x>>1} ^ (x & 1); the odd tail entry has colour 0.
+        if (!((cnt & 1) && a0 == cnt - 1)) t = row_bit(srows, row, a0 >> 1) ^ (a0 & 1);
+        dst[off + q] = static_cast<uint32_t>((t ? a1 : a0) >> 1);
+        dst[off + lc + q] = static_cast<uint32_t>((t ? a0 : a1) >> 1);
+    } else if ((cnt & 1) && q == h) {
+        dst[off + h] = static_cast<uint32_t>(static_cast<uint64_t>(inv[off + cnt - 1]) >> 1);
     }
+    const uint64_t mt = __ballot(t != 0);
+    const uint64_t first = q - lane_id();
+    if (lane_id() == 0 && first < h) trows[row + (first >> 6)] = mt;
 }
 
 // One wave per (group, packet) at depth d on one side.
@@ -224,34 +312,48 @@ __global__ __launch_bounds__(kBlock) void k_pack(const Plan* __restrict__ plan,
 
 // ---------------------------------------------------------------------------
 // Subtrees in LDS
+//
+// One workgroup owns a subtree of at most kSub entries (a node at the fusion
+// depth d0) and runs every remaining level in LDS: node-local permutation
+// (u16, ping-pong), union-find words (u32), and the level's switch values as
+// bit arrays indexed like the permutation.  Work is dealt as (node, 64-entry
+// chunk) items to waves, so each wave stays inside one node and the node's
+// offset comes from a per-level LDS table.
 
-// Node of local entry i at relative level k below a subtree root of n0
-// entries.  Returns false when an ancestor (or the node) is a leaf.
-__device__ __forceinline__ bool locate(uint64_t n0, uint32_t k, uint32_t leaf, uint64_t i,
-                                       uint64_t& pi, uint64_t& off, uint64_t& c) {
-    c = n0;
-    off = 0;
-    pi = 0;
-    for (uint32_t l = 0; l < k; ++l) {
-        if (c <= leaf) return false;
-        const uint64_t lc = (c + 1) >> 1;
-        if (i - off < lc) {
-            c = lc;
-            pi = pi << 1;
+// Number of the first k paths at `level` whose node holds an extra entry
+// (32-bit form of large_before for subtree-local counts).
+__device__ __forceinline__ uint32_t large_before32(uint32_t k, uint32_t m, uint32_t level) {
+    uint32_t included = 0;
+    for (uint32_t bits = level; bits != 0; --bits) {
+        const uint32_t half = 1u << (bits - 1);
+        const uint32_t even = (k + 1) >> 1;
+        if (m <= half) {
+            k = even;
         } else {
-            off += lc;
-            c >>= 1;
-            pi = (pi << 1) | 1;
+            included += even;
+            k >>= 1;
+            m -= half;
         }
     }
-    return c > leaf;
+    return included + ((k != 0 && m != 0) ? 1u : 0u);
 }
 
-// Factoradic rank (internal.hpp:148-167) of one leaf, OR'ed into the blob.
-// Lehmer digit i counts earlier entries smaller than entry i, read off a
-// bitmask of the values seen so far; Horner with radix i+1, i ascending.
-__device__ __forceinline__ void rank_leaf(const Plan& P, uint32_t d, uint64_t path,
-                                          const uint16_t* vals, uint32_t c, uint64_t* blob) {
+__device__ __forceinline__ uint32_t rev32(uint32_t v, uint32_t bits) {
+    return bits == 0 ? 0u : (__brev(v) >> (32 - bits));
+}
+
+// Count of node pi at relative level k below a root of n0 entries.
+__device__ __forceinline__ uint32_t sub_count(uint32_t n0, uint32_t k, uint32_t pi) {
+    const uint32_t m = n0 & ((1u << k) - 1);
+    return (n0 >> k) + (rev32(pi, k) < m ? 1u : 0u);
+}
+
+// Factoradic rank (internal.hpp:148-167) of one leaf, OR'ed into the blob at
+// bit `pos`.  Lehmer digit i counts earlier entries smaller than entry i,
+// read off a bitmask of the values seen so far; Horner with radix i+1,
+// i ascending.
+__device__ __forceinline__ void rank_leaf(uint64_t pos, const uint16_t* vals, uint32_t c,
+                                          uint64_t* blob) {
     uint32_t seen = 0;
     unsigned __int128 r = 0;
     for (uint32_t i = 0; i < c; ++i) {
@@ -261,7 +363,6 @@ __device__ __forceinline__ void rank_leaf(const Plan& P, uint32_t d, uint64_t pa
         r = r * (i + 1) + digit;
     }
     if (rank_bits(c) == 0) return;
-    const uint64_t pos = leaf_section_offset(P) * 8 + leaf_rank_bit(P, d, path);
     const uint64_t lo = static_cast<uint64_t>(r), hi = static_cast<uint64_t>(r >> 64);
     const uint64_t w = pos >> 6;
     const uint32_t sh = pos & 63;
@@ -274,6 +375,25 @@ __device__ __forceinline__ void rank_leaf(const Plan& P, uint32_t d, uint64_t pa
     if (x2) atomicOr(b + w + 2, static_cast<unsigned long long>(x2));
 }
 
+// Walk state at the subtree root, per side, plus the separate-placement rank
+// offset there: every address below continues from it.
+struct SubtreeRoot {
+    Walk walk[2];
+    uint64_t middle;
+};
+
+// Separate-placement rank offset of node (d, path), continuing from depth
+// `from` (benes.cpp:317-319: a right child starts after its left sibling's
+// ranks).
+__device__ __forceinline__ uint64_t middle_from(const Plan& P, uint64_t mid, uint32_t from,
+                                                uint32_t d, uint64_t path) {
+    for (uint32_t i = from + 1; i <= d; ++i) {
+        const uint64_t here = path >> (d - i);
+        if (here & 1) mid += P.middle_bits[i][node_large(P.n, i, here - 1) ? 1 : 0];
+    }
+    return mid;
+}
+
 // Writes the switch bits of every internal node at relative level k of a
 // subtree (absolute depth d0 + k) into the blob.  Bits are produced packet by
 // packet: within a cluster group, one packet's share of a level is the
@@ -281,15 +401,16 @@ __device__ __forceinline__ void rank_leaf(const Plan& P, uint32_t d, uint64_t pa
 // (internal.hpp:604-649), so a run of consecutive nodes is one contiguous
 // stretch of the blob, written as whole words.  `sbits`/`tbits` hold the
 // level's switch values at the position of each node's pair in LDS.
-__device__ __forceinline__ void emit_level_runs(const Plan& P, uint32_t d0, uint64_t b0,
-                                                uint64_t n0, uint32_t k,
-                                                const uint64_t* sbits, const uint64_t* tbits,
-                                                const uint16_t* noff, uint64_t* blob) {
+__device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot& root,
+                                                uint32_t d0, uint64_t b0, uint32_t n0,
+                                                uint32_t k, const uint64_t* sbits,
+                                                const uint64_t* tbits, const uint16_t* noff,
+                                                uint64_t* blob) {
     const uint32_t d = d0 + k;
     const uint32_t L = P.leaf;
     const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    const uint64_t cmax = (n0 + ((uint64_t{1} << k) - 1)) >> k;
-    const uint64_t maxpairs = cmax >> 1;
+    const uint32_t cmax = (n0 + ((1u << k) - 1)) >> k;
+    const uint32_t maxpairs = cmax >> 1;
     for (int side = 0; side < 2; ++side) {
         const SideDepth& sd = P.side[side][d];
         const uint32_t r = sd.relative_level, g = sd.group_depth;
@@ -300,33 +421,38 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, uint32_t d0, uint
         const uint64_t secbase = (side == 0 ? P.input_offset : P.output_offset) * 8;
         // Sets of consecutive nodes that share one cluster group.
         const bool one_set = R <= d0;
-        const uint64_t nsets = one_set ? 1 : (uint64_t{1} << (R - d0));
-        const uint64_t set_nodes = one_set ? (uint64_t{1} << k) : (uint64_t{1} << r);
-        const uint64_t packets = (maxpairs + span - 1) >> span_log2;
-        for (uint64_t job = wave; job < nsets * packets; job += nwaves) {
-            const uint64_t set = job / packets, pk = job % packets;
-            const uint64_t first = set * set_nodes;
-            const uint64_t minc = one_set ? (n0 >> k) : (node_count(n0, R - d0, set) >> r);
-            const uint64_t first_pair = pk << span_log2;
+        const uint32_t nsets = one_set ? 1u : (1u << (R - d0));
+        const uint32_t set_nodes = one_set ? (1u << k) : (1u << r);
+        const uint32_t packets = static_cast<uint32_t>((maxpairs + span - 1) >> span_log2);
+        for (uint32_t job = wave; job < nsets * packets; job += nwaves) {
+            const uint32_t set = job / packets, pk = job - set * packets;
+            const uint32_t first = set * set_nodes;
+            const uint32_t minc = one_set ? (n0 >> k) : (sub_count(n0, R - d0, set) >> r);
+            const uint64_t first_pair = static_cast<uint64_t>(pk) << span_log2;
             if (minc > L && (minc >> 1) >= first_pair + span) {
                 // Every node fills its share of this packet: one run.
-                const SideAddr a = node_address(P, side, d, (b0 << k) | first);
-                wave_write_run(blob, secbase + bit_of(a, first_pair), set_nodes << span_log2,
-                               [&](uint64_t q) {
-                                   const uint64_t pos = noff[first + (q >> span_log2)] +
-                                                        first_pair + (q & (span - 1));
+                const uint64_t path = (b0 << k) | first;
+                const SideAddr a = address_at(P, walk_from(P, side, root.walk[side], d0, d, path),
+                                              d, path);
+                wave_write_run(blob, secbase + bit_of(a, first_pair),
+                               static_cast<uint64_t>(set_nodes) << span_log2, [&](uint64_t q) {
+                                   const uint32_t pos =
+                                       noff[first + static_cast<uint32_t>(q >> span_log2)] +
+                                       static_cast<uint32_t>(first_pair + (q & (span - 1)));
                                    return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
                                });
             } else {
-                for (uint64_t pi = first; pi < first + set_nodes; ++pi) {
-                    const uint64_t c = node_count(n0, k, pi);
+                for (uint32_t pi = first; pi < first + set_nodes; ++pi) {
+                    const uint32_t c = sub_count(n0, k, pi);
                     if (c <= L) continue;
                     const uint64_t len = packet_share(c, pk, span_log2, L);
                     if (len == 0) continue;
-                    const SideAddr a = node_address(P, side, d, (b0 << k) | pi);
-                    const uint64_t base = noff[pi] + first_pair;
+                    const uint64_t path = (b0 << k) | pi;
+                    const SideAddr a = address_at(
+                        P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
+                    const uint32_t base = noff[pi] + static_cast<uint32_t>(first_pair);
                     wave_write_run(blob, secbase + bit_of(a, first_pair), len, [&](uint64_t q) {
-                        const uint64_t pos = base + q;
+                        const uint32_t pos = base + static_cast<uint32_t>(q);
                         return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
                     });
                 }
@@ -343,77 +469,116 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const V* __restrict__ s
     if (*err) return;
     __shared__ uint16_t buf[2][kSub];
     __shared__ uint32_t uf[kSub];
-    __shared__ uint64_t sbits[kSub / 64];
-    __shared__ uint64_t tbits[kSub / 64];
-    __shared__ uint16_t noff[kSub / 2];
+    __shared__ uint64_t sbits[kSub / 64 + 2];
+    __shared__ uint64_t tbits[kSub / 64 + 2];
+    __shared__ uint16_t noff[kSub / 2 + 1];
+    __shared__ SubtreeRoot root;
     const Plan& P = *plan;
     const uint32_t L = P.leaf;
     const uint64_t b0 = blockIdx.x;
-    const uint64_t n0 = node_count(P.n, d0, b0);
+    const uint32_t n0 = static_cast<uint32_t>(node_count(P.n, d0, b0));
     const uint64_t base = node_offset(P.n, d0, b0);
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nthreads = blockDim.x, nwaves = blockDim.x >> 6;
-    for (uint64_t i = tid; i < n0; i += nthreads) buf[0][i] = static_cast<uint16_t>(src[base + i]);
+    if (tid == 0) {
+        root.walk[0] = walk_to(P, 0, d0, b0);
+        root.walk[1] = walk_to(P, 1, d0, b0);
+        root.middle = middle_from(P, 0, 0, d0, b0);
+    }
+    for (uint32_t i = tid; i < n0; i += nthreads) buf[0][i] = static_cast<uint16_t>(src[base + i]);
     __syncthreads();
+    const int rank_side = P.placement == 1 ? 0 : 1;
+    const uint64_t rank_section = leaf_section_offset(P) * 8;
     int cur = 0;
     for (uint32_t k = 0;; ++k) {
         const uint32_t d = d0 + k;
         uint16_t* perm = buf[cur];
         uint16_t* next = buf[cur ^ 1];
-        const uint64_t cmax = (n0 + ((uint64_t{1} << k) - 1)) >> k;
-        const uint64_t cmin = n0 >> k;
-        const uint64_t nodes = uint64_t{1} << k;
-        if (cmin <= L) {
-            for (uint64_t pi = tid; pi < nodes; pi += nthreads) {
-                const uint64_t c = node_count(n0, k, pi);
+        const uint32_t nodes = 1u << k;
+        const uint32_t q = n0 >> k, m = n0 & (nodes - 1);
+        const uint32_t cmax = q + (m != 0 ? 1u : 0u);
+        // Node offsets of this level (leaves included).
+        for (uint32_t pi = tid; pi < nodes; pi += nthreads) {
+            noff[pi] = static_cast<uint16_t>(pi * q + large_before32(pi, m, k));
+        }
+        __syncthreads();
+        // Leaves whose parent was internal: rank them now.
+        if (q <= L) {
+            for (uint32_t pi = tid; pi < nodes; pi += nthreads) {
+                const uint32_t c = sub_count(n0, k, pi);
                 if (c > L) continue;
-                if (k > 0 && node_count(n0, k - 1, pi >> 1) <= L) continue;
-                const uint64_t off = node_offset(n0, k, pi);
-                rank_leaf(P, d, (b0 << k) | pi, perm + off, static_cast<uint32_t>(c), blob);
+                if (k > 0 && sub_count(n0, k - 1, pi >> 1) <= L) continue;
+                const uint64_t path = (b0 << k) | pi;
+                uint64_t bit;
+                if (P.placement == 0) {
+                    bit = middle_from(P, root.middle, d0, d, path);
+                } else {
+                    bit = rank_bit_at(walk_from(P, rank_side, root.walk[rank_side], d0, d, path),
+                                      L);
+                }
+                rank_leaf(rank_section + bit, perm + noff[pi], c, blob);
             }
         }
         if (cmax <= L) break;
-        for (uint64_t i = tid; i < n0; i += nthreads) uf[i] = UfWord<uint32_t>::root;
-        __syncthreads();
-        for (uint64_t i = tid; i < n0; i += nthreads) {
-            uint64_t pi, off, c;
-            if (!locate(n0, k, L, i, pi, off, c)) continue;
-            const uint64_t x = i - off, v = perm[i];
-            if ((c & 1) && v == c - 1) continue;
-            uf_union<GroupScope>(uf + off, input_slot<uint32_t>(x, c),
-                                 output_pair_slot<uint32_t>(v >> 1, c),
-                                 static_cast<uint32_t>((x ^ v) & 1));
+        for (uint32_t i = tid; i < n0; i += nthreads) uf[i] = UfWord<uint32_t>::root;
+        for (uint32_t i = tid; i < kSub / 64 + 2; i += nthreads) {
+            sbits[i] = 0;
+            tbits[i] = 0;
         }
         __syncthreads();
-        for (uint64_t wbase = static_cast<uint64_t>(wave) * 64; wbase < n0; wbase += nwaves * 64) {
-            const uint64_t i = wbase + lane;
+        // Union: items are (node, 64-entry chunk).
+        const uint32_t cpn = (cmax + 63) >> 6;
+        for (uint32_t item = wave; item < nodes * cpn; item += nwaves) {
+            const uint32_t pi = item / cpn, chunk = item - pi * cpn;
+            const uint32_t c = sub_count(n0, k, pi);
+            if (c <= L) continue;
+            const uint32_t x = chunk * 64 + lane;
+            if (x >= c) continue;
+            const uint32_t off = noff[pi];
+            const uint32_t v = perm[off + x];
+            if ((c & 1) && v == c - 1) continue;
+            uf_union_edge<GroupForest>(uf + off, input_slot<uint32_t>(x, c),
+                                       output_pair_slot<uint32_t>(v >> 1, c), (x ^ v) & 1u);
+        }
+        __syncthreads();
+        // Resolve and route: items are (node, 64-pair chunk); the odd tail
+        // entry rides with pair index h.
+        const uint32_t ppn = ((cmax >> 1) + 1 + 63) >> 6;
+        for (uint32_t item = wave; item < nodes * ppn; item += nwaves) {
+            const uint32_t pi = item / ppn, chunk = item - pi * ppn;
+            const uint32_t c = sub_count(n0, k, pi);
+            if (c <= L) continue;  // wave-uniform
+            const uint32_t off = noff[pi], h = c >> 1;
+            const uint32_t p = chunk * 64 + lane;
             uint32_t s = 0, t = 0;
-            uint64_t pi, off, c;
-            if (i < n0 && locate(n0, k, L, i, pi, off, c)) {
-                const uint64_t p = i - off, h = c >> 1;
-                if (p < h) {
-                    uint32_t* u = uf + off;
-                    uf_find<GroupScope>(u, input_pair_slot<uint32_t>(p, c), s);
-                    uf_find<GroupScope>(u, output_pair_slot<uint32_t>(p, c), t);
-                    const uint32_t a0 = perm[off + 2 * p], a1 = perm[off + 2 * p + 1];
-                    const uint64_t lc = h + (c & 1);
-                    next[off + p] = static_cast<uint16_t>((s ? a1 : a0) >> 1);
-                    next[off + lc + p] = static_cast<uint16_t>((s ? a0 : a1) >> 1);
-                } else if ((c & 1) && p == h) {
-                    next[off + h] = static_cast<uint16_t>(perm[off + c - 1] >> 1);
-                }
+            if (p < h) {
+                uint32_t* u = uf + off;
+                uf_find<GroupForest>(u, input_pair_slot<uint32_t>(p, c), s);
+                uf_find<GroupForest>(u, output_pair_slot<uint32_t>(p, c), t);
+                const uint32_t a0 = perm[off + 2 * p], a1 = perm[off + 2 * p + 1];
+                const uint32_t lc = h + (c & 1);
+                next[off + p] = static_cast<uint16_t>((s ? a1 : a0) >> 1);
+                next[off + lc + p] = static_cast<uint16_t>((s ? a0 : a1) >> 1);
+            } else if ((c & 1) && p == h) {
+                next[off + h] = static_cast<uint16_t>(perm[off + c - 1] >> 1);
             }
             const uint64_t ms = __ballot(s != 0), mt = __ballot(t != 0);
             if (lane == 0) {
-                sbits[wbase >> 6] = ms;
-                tbits[wbase >> 6] = mt;
+                const uint32_t pos = off + chunk * 64, w = pos >> 6, sh = pos & 63;
+                if (ms) {
+                    atomicOr(reinterpret_cast<unsigned long long*>(sbits + w), ms << sh);
+                    if (sh) atomicOr(reinterpret_cast<unsigned long long*>(sbits + w + 1),
+                                     ms >> (64 - sh));
+                }
+                if (mt) {
+                    atomicOr(reinterpret_cast<unsigned long long*>(tbits + w), mt << sh);
+                    if (sh) atomicOr(reinterpret_cast<unsigned long long*>(tbits + w + 1),
+                                     mt >> (64 - sh));
+                }
             }
         }
-        for (uint64_t pi = tid; pi < nodes; pi += nthreads) {
-            noff[pi] = static_cast<uint16_t>(node_offset(n0, k, pi));
-        }
         __syncthreads();
-        emit_level_runs(P, d0, b0, n0, k, sbits, tbits, noff, blob);
+        emit_level_runs(P, root, d0, b0, n0, k, sbits, tbits, noff, blob);
         __syncthreads();
         cur ^= 1;
     }

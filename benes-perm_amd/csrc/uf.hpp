@@ -12,9 +12,25 @@
 // (benes.cpp:278-297: the odd tail, else the component's smallest input), and a
 // slot's parity to its root is exactly its canonical switch value.
 //
-// Path halving keeps every stored word a true parity relation to an ancestor:
-// x -> gp with parity p(x->p) ^ p(p->gp), published by compare-and-swap so a
-// concurrent hook or halving of x is never overwritten.
+// Why cached loads and plain stores are enough.  Every word ever stored is a
+// true parity relation from a slot to a strictly smaller slot: a hook links a
+// root to a smaller slot, and path halving links a non-root to one of its
+// ancestors.  A slot that has a parent never becomes a root again, so a load
+// that returns an out-of-date word still returns a true relation to a smaller
+// slot.  Therefore
+//   * walks always terminate (indices strictly decrease) and their parities are
+//     right, whatever mix of fresh and stale words they read;
+//   * only the hook must be atomic: a compare-and-swap that expects the
+//     all-ones word, at device scope, so exactly one of any competing hooks of
+//     a root succeeds;
+//   * a failed hook returns the root's true parent, from which the walk
+//     resumes, so every retry moves to a strictly smaller slot;
+//   * halving writes only to slots that are no longer roots, which no hook
+//     touches, so a plain store is safe; losing one to a concurrent halving
+//     only leaves a longer (still true) path.
+// Two endpoints that reach the same apparent root are truly connected, since
+// the relations they followed are facts.  After the last union the forest is
+// static apart from halving, so every later walk reaches the true root.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,49 +49,62 @@ struct UfWord<uint64_t> {
     static constexpr uint64_t root = ~uint64_t{0};
 };
 
-// Memory-scope policies: device-wide parent arrays (global memory shared by
-// many workgroups, possibly on different XCDs) and workgroup-local ones (LDS).
-struct AgentScope {
-    template <typename W>
-    static __device__ __forceinline__ W load(W* p) {
-        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    template <typename W>
-    static __device__ __forceinline__ bool cas(W* p, W expected, W desired) {
-        return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
-                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-};
-
-struct GroupScope {
+// Parent arrays in HBM, shared by workgroups on every XCD.
+struct DeviceForest {
     template <typename W>
     static __device__ __forceinline__ W load(W* p) {
         return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     template <typename W>
-    static __device__ __forceinline__ bool cas(W* p, W expected, W desired) {
-        return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
-                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    static __device__ __forceinline__ void store(W* p, W v) {
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    // Returns the word found (the all-ones word when the hook succeeded).
+    template <typename W>
+    static __device__ __forceinline__ W hook(W* p, W desired) {
+        W expected = UfWord<W>::root;
+        __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return expected;
     }
 };
 
-// Root of x and the parity of x relative to it.
-template <typename Scope, typename W>
+// Parent arrays in LDS, private to one workgroup.
+struct GroupForest {
+    template <typename W>
+    static __device__ __forceinline__ W load(W* p) {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    template <typename W>
+    static __device__ __forceinline__ void store(W* p, W v) {
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    template <typename W>
+    static __device__ __forceinline__ W hook(W* p, W desired) {
+        W expected = UfWord<W>::root;
+        __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return expected;
+    }
+};
+
+// Root of x and the parity of x relative to it (path halving on the way).
+template <typename F, typename W>
 __device__ __forceinline__ W uf_find(W* uf, W x, uint32_t& parity) {
     constexpr W kRoot = UfWord<W>::root;
     uint32_t par = 0;
     while (true) {
-        const W w = Scope::load(uf + x);
+        const W w = F::load(uf + x);
         if (w == kRoot) break;
         const W p = w >> 1;
-        const W w2 = Scope::load(uf + p);
+        const W w2 = F::load(uf + p);
         if (w2 == kRoot) {
             par ^= static_cast<uint32_t>(w & 1);
             x = p;
             break;
         }
         const W hop = (w ^ w2) & 1;
-        Scope::cas(uf + x, w, (w2 & ~W{1}) | hop);
+        F::store(uf + x, (w2 & ~W{1}) | hop);
         par ^= static_cast<uint32_t>(hop);
         x = w2 >> 1;
     }
@@ -83,20 +112,53 @@ __device__ __forceinline__ W uf_find(W* uf, W x, uint32_t& parity) {
     return x;
 }
 
-// Records value(u) ^ value(v) == label.
-template <typename Scope, typename W>
-__device__ __forceinline__ void uf_union(W* uf, W u, W v, uint32_t label) {
+// Records value(u) ^ value(v) == label, given u's root (ru, parity pu) and
+// v's root (rv, parity pv).
+template <typename F, typename W>
+__device__ __forceinline__ void uf_link(W* uf, W ru, uint32_t pu, W rv, uint32_t pv,
+                                        uint32_t label) {
     constexpr W kRoot = UfWord<W>::root;
-    while (true) {
-        uint32_t pu, pv;
-        const W ru = uf_find<Scope>(uf, u, pu);
-        const W rv = uf_find<Scope>(uf, v, pv);
-        if (ru == rv) return;
+    while (ru != rv) {
+        const bool u_high = ru > rv;
+        const W hi = u_high ? ru : rv, lo = u_high ? rv : ru;
         const W rel = static_cast<W>((pu ^ pv ^ label) & 1);
-        const W hi = ru > rv ? ru : rv;
-        const W lo = ru > rv ? rv : ru;
-        if (Scope::cas(uf + hi, kRoot, (lo << 1) | rel)) return;
+        const W found = F::hook(uf + hi, (lo << 1) | rel);
+        if (found == kRoot) return;
+        // hi already has a parent: continue from it.
+        uint32_t q;
+        const W r = uf_find<F>(uf, found >> 1, q);
+        q ^= static_cast<uint32_t>(found & 1);
+        if (u_high) {
+            ru = r;
+            pu ^= q;
+        } else {
+            rv = r;
+            pv ^= q;
+        }
     }
+}
+
+// Union of input slot u and output slot v (u < v).  Output slots never get
+// children (every hook links a larger root under a smaller one, and an input
+// slot is smaller than every output slot), so when v is still a root the
+// edge is settled by hooking v straight under u.
+template <typename F, typename W>
+__device__ __forceinline__ void uf_union_edge(W* uf, W u, W v, uint32_t label) {
+    const W found = F::hook(uf + v, (u << 1) | static_cast<W>(label & 1));
+    if (found == UfWord<W>::root) return;
+    uint32_t pu, pv;
+    const W ru = uf_find<F>(uf, u, pu);
+    const W rv = uf_find<F>(uf, found >> 1, pv);
+    uf_link<F>(uf, ru, pu, rv, pv ^ static_cast<uint32_t>(found & 1), label);
+}
+
+// Union of two input slots (the input-pair forest used on HBM levels).
+template <typename F, typename W>
+__device__ __forceinline__ void uf_union_slots(W* uf, W a, W b, uint32_t label) {
+    uint32_t pa, pb;
+    const W ra = uf_find<F>(uf, a, pa);
+    const W rb = uf_find<F>(uf, b, pb);
+    uf_link<F>(uf, ra, pa, rb, pb, label);
 }
 
 // Slot numbering inside a node of `count` entries (h = count/2 pairs).
