@@ -205,17 +205,24 @@ void enqueue_pipeline(Workspace& ws, const V* src, const Layout& lay, Profiler& 
         // Input slots of a node number ceil(c/2) < 2^31 for 32-bit words.
         const bool wide = cmax / 2 + 1 >= (uint64_t{1} << 31);
         ws.uf.ensure(n * (wide ? 8 : 4));
-        BH_CHECK(hipMemsetAsync(ws.uf.ptr, 0xFF, n * (wide ? 8 : 4), st));
+        if (cmax > kLevelLds) BH_CHECK(hipMemsetAsync(ws.uf.ptr, 0xFF, n * (wide ? 8 : 4), st));
         const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
         const dim3 grid(static_cast<uint32_t>(chunks * nodes));
         uint32_t* pdst = ws.level[next].as<uint32_t>();
         uint32_t* idst = ws.ilevel[next].as<uint32_t>();
         uint64_t* srows = ws.srows.as<uint64_t>();
         uint64_t* trows = ws.trows.as<uint64_t>();
+        const bool in_lds = cmax <= kLevelLds;
         auto run = [&](auto psrc, auto isrc, auto uf) {
             using VS = std::remove_const_t<std::remove_pointer_t<decltype(psrc)>>;
             using VI = std::remove_const_t<std::remove_pointer_t<decltype(isrc)>>;
             using W = std::remove_pointer_t<decltype(uf)>;
+            if (in_lds) {
+                BH_LAUNCH(prof, "k_level_lds", (k_level_lds<VS, VI>),
+                          dim3(static_cast<uint32_t>(nodes)), dim3(kLevelThreads), 0, st, psrc,
+                          isrc, pdst, idst, desc, srows, trows, err);
+                return;
+            }
             BH_LAUNCH(prof, "k_union", (k_union<VI, W>), grid, dim3(kBlock), 0, st, isrc, uf, desc,
                       nodes, chunks, err);
             BH_LAUNCH(prof, "k_route_inputs", (k_route_inputs<VS, W>), grid, dim3(kBlock), 0, st,

@@ -146,9 +146,12 @@ __device__ __forceinline__ void block_to_node(uint64_t nodes, uint64_t chunks, u
         chunk = b % chunks;
         return;
     }
-    const uint64_t xcd = b % kXcds, local = b / kXcds;
-    node = (local / chunks) * kXcds + xcd;
-    chunk = local % chunks;
+    // Consecutive blocks of one XCD visit its nodes in turn, so the blocks in
+    // flight on an XCD spread over all of its nodes instead of crowding one
+    // node's hottest union-find words.
+    const uint64_t xcd = b % kXcds, local = b / kXcds, per_xcd = nodes / kXcds;
+    node = (local % per_xcd) * kXcds + xcd;
+    chunk = local / per_xcd;
 }
 
 __global__ void k_describe(const Plan* __restrict__ plan, uint32_t d, uint64_t row_words,
@@ -263,6 +266,79 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const VI* __restrict__
     const uint64_t mt = __ballot(t != 0);
     const uint64_t first = q - lane_id();
     if (lane_id() == 0 && first < h) trows[row + (first >> 6)] = mt;
+}
+
+// One workgroup per node of at most kLevelLds entries: the same three steps
+// as k_union / k_route_inputs / k_route_outputs, with the node's union-find
+// words and its input switch values in LDS instead of HBM.  A node of 2^16
+// entries has 2^15 input slots: 128 KiB of parent words.
+constexpr uint32_t kLevelLdsLog2 = 16;
+constexpr uint32_t kLevelLds = 1u << kLevelLdsLog2;
+constexpr uint32_t kLevelThreads = 1024;
+
+template <typename V, typename VI>
+__global__ __launch_bounds__(kLevelThreads) void k_level_lds(
+    const V* __restrict__ psrc, const VI* __restrict__ isrc, uint32_t* __restrict__ pdst,
+    uint32_t* __restrict__ idst, const NodeDesc* __restrict__ desc, uint64_t* __restrict__ srows,
+    uint64_t* __restrict__ trows, const int* __restrict__ err) {
+    if (*err) return;
+    __shared__ uint32_t uf[kLevelLds / 2 + 1];
+    __shared__ uint64_t sbits[kLevelLds / 128 + 1];
+    const uint64_t node = blockIdx.x;
+    const uint64_t off = desc[node].off, row = desc[node].row;
+    const uint32_t cnt = static_cast<uint32_t>(desc[node].cnt);
+    const uint32_t h = cnt >> 1, odd = cnt & 1, lc = h + odd;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t nthreads = blockDim.x, nwaves = blockDim.x >> 6;
+    for (uint32_t i = tid; i < lc; i += nthreads) uf[i] = UfWord<uint32_t>::root;
+    __syncthreads();
+    for (uint32_t q = tid; q < h; q += nthreads) {
+        const uint32_t a = static_cast<uint32_t>(isrc[off + 2 * q]);
+        const uint32_t b = static_cast<uint32_t>(isrc[off + 2 * q + 1]);
+        const uint32_t sa = static_cast<uint32_t>(in_slot(a, cnt));
+        const uint32_t sb = static_cast<uint32_t>(in_slot(b, cnt));
+        if (sa != sb) uf_union_slots<GroupForest>(uf, sa, sb, 1u ^ ((a ^ b) & 1u));
+    }
+    __syncthreads();
+    // Input switch values: route P, keep s in LDS and in the node's row.
+    for (uint32_t base = wave * 64; base < lc; base += nwaves * 64) {
+        const uint32_t p = base + lane;
+        uint32_t s = 0;
+        if (p < h) {
+            uf_find<GroupForest>(uf, p + odd, s);
+            const uint64_t a0 = static_cast<uint64_t>(psrc[off + 2 * p]);
+            const uint64_t a1 = static_cast<uint64_t>(psrc[off + 2 * p + 1]);
+            pdst[off + p] = static_cast<uint32_t>((s ? a1 : a0) >> 1);
+            pdst[off + lc + p] = static_cast<uint32_t>((s ? a0 : a1) >> 1);
+        } else if (odd && p == h) {
+            pdst[off + h] = static_cast<uint32_t>(static_cast<uint64_t>(psrc[off + cnt - 1]) >> 1);
+        }
+        const uint64_t ms = __ballot(s != 0);
+        if (lane == 0) {
+            sbits[base >> 6] = ms;
+            if (base < h) srows[row + (base >> 6)] = ms;
+        }
+    }
+    __syncthreads();
+    // Output switch values t_q = colour(I[2q]); route the inverse.
+    for (uint32_t base = wave * 64; base < lc; base += nwaves * 64) {
+        const uint32_t q = base + lane;
+        uint32_t t = 0;
+        if (q < h) {
+            const uint32_t a0 = static_cast<uint32_t>(isrc[off + 2 * q]);
+            const uint32_t a1 = static_cast<uint32_t>(isrc[off + 2 * q + 1]);
+            if (!(odd && a0 == cnt - 1)) {
+                const uint32_t pa = a0 >> 1;
+                t = static_cast<uint32_t>((sbits[pa >> 6] >> (pa & 63)) & 1) ^ (a0 & 1);
+            }
+            idst[off + q] = (t ? a1 : a0) >> 1;
+            idst[off + lc + q] = (t ? a0 : a1) >> 1;
+        } else if (odd && q == h) {
+            idst[off + h] = static_cast<uint32_t>(isrc[off + cnt - 1]) >> 1;
+        }
+        const uint64_t mt = __ballot(t != 0);
+        if (lane == 0 && base < h) trows[row + (base >> 6)] = mt;
+    }
 }
 
 // One wave per (group, packet) at depth d on one side.
