@@ -35,6 +35,9 @@ EXPORTS = (
     "benes_hip_available",
     "benes_hip_compress",
     "benes_hip_compress_device",
+    "benes_hip_batch_layout",
+    "benes_hip_compress_batch",
+    "benes_hip_compress_batch_device",
     "benes_hip_encoded_size",
     "benes_hip_last_device_seconds",
     "benes_hip_set_profiling",
@@ -54,6 +57,21 @@ class _Format(ctypes.Structure):
         ("n_output_clusters", ctypes.c_uint8),
         ("input_clusters", ctypes.POINTER(ctypes.c_uint8)),
         ("output_clusters", ctypes.POINTER(ctypes.c_uint8)),
+    ]
+
+
+class _Item(ctypes.Structure):
+    _fields_ = [
+        ("perm", ctypes.c_void_p),
+        ("n", ctypes.c_uint64),
+        ("width_bytes", ctypes.c_uint32),
+        ("status", ctypes.c_int32),
+        ("format", _Format),
+        ("out", ctypes.c_void_p),
+        ("out_capacity", ctypes.c_uint64),
+        ("out_offset", ctypes.c_uint64),
+        ("out_size", ctypes.c_uint64),
+        ("message", ctypes.c_char * 96),
     ]
 
 
@@ -111,6 +129,11 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_compress_device.argtypes = [vp, u64, u32, fp, vp, u64, ctypes.POINTER(u64),
                                               vp, cp, sz]
     lib.benes_hip_encoded_size.argtypes = [u64, fp, ctypes.POINTER(u64), cp, sz]
+    ip = ctypes.POINTER(_Item)
+    lib.benes_hip_batch_layout.argtypes = [ip, u64, ctypes.POINTER(u64)]
+    lib.benes_hip_compress_batch.argtypes = [ip, u64, ctypes.POINTER(ctypes.c_double), cp, sz]
+    lib.benes_hip_compress_batch_device.argtypes = [ip, u64, vp, u64, vp,
+                                                    ctypes.POINTER(ctypes.c_double), cp, sz]
     lib.benes_hip_last_device_seconds.restype = ctypes.c_double
     lib.benes_hip_set_profiling.argtypes = [ctypes.c_int]
     lib.benes_hip_set_profiling.restype = None
@@ -201,6 +224,72 @@ def compress_device(d_perm_ptr: int, n: int, width: int, d_out_ptr: int, capacit
                                              ctypes.c_void_p(stream or 0), err, 256)
     _raise(rc, err)
     return size.value
+
+
+def _items(perm_ptrs, ns, widths, fmts):
+    arr = (_Item * len(ns))()
+    keep = []
+    for i, (ptr, n, w, f) in enumerate(zip(perm_ptrs, ns, widths, fmts)):
+        cf, k = _format(f)
+        keep.append(k)
+        arr[i].perm = ptr
+        arr[i].n = n
+        arr[i].width_bytes = w
+        arr[i].format = cf
+    return arr, keep
+
+
+def _item_error(item) -> Exception | None:
+    if item.status == OK:
+        return None
+    return _ERRORS.get(item.status, RuntimeError)(item.message.decode(errors="replace"))
+
+
+def batch_layout(ns, widths, fmts):
+    """(out_offsets, out_sizes, total_bytes) of a device batch output."""
+    arr, keep = _items([None] * len(ns), ns, widths, fmts)
+    total = ctypes.c_uint64()
+    library().benes_hip_batch_layout(arr, len(ns), ctypes.byref(total))
+    return [it.out_offset for it in arr], [it.out_size for it in arr], total.value
+
+
+def compress_batch(perms, fmts=None):
+    """Compresses many host permutations in one call.  Returns, per item, the
+    blob bytes or the exception that item raised."""
+    fmts = fmts or [None] * len(perms)
+    perms = [np.ascontiguousarray(p) for p in perms]
+    arr, keep = _items([p.ctypes.data for p in perms], [p.size for p in perms],
+                       [p.dtype.itemsize for p in perms], fmts)
+    sizes = []
+    for it in arr:
+        # out_size is filled by the call; allocate from the layout first
+        sizes.append(0)
+    offs, sizes, _ = batch_layout([p.size for p in perms], [p.dtype.itemsize for p in perms], fmts)
+    outs = [np.zeros(max(1, s), np.uint8) for s in sizes]
+    for it, o in zip(arr, outs):
+        it.out = o.ctypes.data
+        it.out_capacity = o.size
+    err = ctypes.create_string_buffer(256)
+    secs = ctypes.c_double()
+    _raise(library().benes_hip_compress_batch(arr, len(perms), ctypes.byref(secs), err, 256), err)
+    result = []
+    for it, o in zip(arr, outs):
+        e = _item_error(it)
+        result.append(e if e is not None else o[:it.out_size].tobytes())
+    return result
+
+
+def compress_batch_device(perm_ptrs, ns, widths, fmts, d_out_ptr, capacity, stream=None):
+    """Device batch: inputs and the packed output are device memory.  Returns
+    (offsets, sizes, errors, device_seconds)."""
+    arr, keep = _items(perm_ptrs, ns, widths, fmts)
+    err = ctypes.create_string_buffer(256)
+    secs = ctypes.c_double()
+    _raise(library().benes_hip_compress_batch_device(arr, len(ns), ctypes.c_void_p(d_out_ptr),
+                                                     capacity, ctypes.c_void_p(stream or 0),
+                                                     ctypes.byref(secs), err, 256), err)
+    return ([it.out_offset for it in arr], [it.out_size for it in arr],
+            [_item_error(it) for it in arr], secs.value)
 
 
 def last_device_seconds() -> float:

@@ -2,19 +2,23 @@
 // Host orchestration of the compact-v1 compression path on one MI355X, and
 // the C ABI declared in include/benes_hip.h.
 //
-// One call = validate on device, then for each global level
-// describe -> union -> resolve -> pack(in) -> pack(out), then one
-// k_subtree launch for everything below the fusion depth, all queued on one
-// stream; the host only builds the per-depth Plan (a few KB) and reads back a
-// single error word at the end.
+// A call compresses a batch (a plain compress is a batch of one): every item
+// gets a slice of shared HBM arrays, and each step below is one launch over
+// all items that take part in it, driven by a table of per-item segments:
+//   validation + inverse  -> per depth above the fusion depth:
+//   describe -> union -> route inputs -> route outputs (or one LDS kernel per
+//   node up to 2^16 entries) -> pack both sides  -> one subtree launch.
+// The host builds the per-item Plans and all segment tables up front, uploads
+// them in one copy, queues everything on one stream and reads back one error
+// word per item at the end.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/benes_hip.h"
@@ -72,14 +76,15 @@ struct DevBuf {
     }
 };
 
-// Per-device scratch reused across calls (calls on one device serialise on
-// its mutex; separate devices run independently).
+// Device scratch of one stream, reused across calls.
 struct Workspace {
     std::mutex mu;
     int device = -1;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-    DevBuf input, level[2], ilevel[2], inv0, uf, srows, trows, desc, plan, err, blob;
+    DevBuf input, level[2], ilevel[2], inv0, uf, srows, trows, desc, plans, items, segs, errs,
+        blob;
+    std::vector<int> host_errs;
     void init(int dev) {
         if (stream) return;
         device = dev;
@@ -99,11 +104,13 @@ Workspace& workspace_for(int dev) {
 }
 
 uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+uint64_t round8(uint64_t a) { return (a + 7) & ~uint64_t{7}; }
 
-uint32_t fusion_depth(uint64_t n) {
+// First depth whose largest node fits one LDS subtree.
+uint32_t fusion_depth(uint64_t n, uint32_t routing_depth) {
     uint32_t d = 0;
     while (ceil_div(n, uint64_t{1} << d) > kSub) ++d;
-    return d;
+    return std::min(d, routing_depth);
 }
 
 struct Profiler {
@@ -144,142 +151,281 @@ struct Profiler {
     }
 };
 
-#define BH_LAUNCH(prof, name, ...)      \
-    do {                                \
-        (prof).begin(name);             \
+#define BH_LAUNCH(prof, name, ...)       \
+    do {                                 \
+        (prof).begin(name);              \
         hipLaunchKernelGGL(__VA_ARGS__); \
-        (prof).end();                   \
+        (prof).end();                    \
     } while (0)
 
-// Queues the whole pipeline on ws.stream.  `src` holds n entries of V on the
-// device; the blob (rounded up to whole words) lands in ws.blob.
-template <typename V>
-void enqueue_pipeline(Workspace& ws, const V* src, const Layout& lay, Profiler& prof) {
-    const Plan& plan = lay.plan;
-    const uint64_t n = plan.n;
+// One permutation of a batch as the host sees it.
+struct Job {
+    const void* dsrc = nullptr;  // device pointer to its entries
+    uint64_t n = 0;
+    uint32_t width = 4;
+    Layout lay;
+    uint64_t base = 0;           // entry offset in the shared arrays
+    uint64_t blob = 0;           // word offset in the shared blob
+    uint32_t df = 0;
+};
+
+// A launch: an index range of the uploaded segment table and its grid.
+struct Launch {
+    uint32_t first = 0, count = 0;
+    uint64_t blocks = 0;
+};
+
+// Collects the segments of one launch.
+struct SegList {
+    std::vector<Seg> segs;
+    uint64_t blocks = 0;
+    void add(uint32_t item, uint64_t blocks_of_item, uint64_t nodes, uint64_t chunks,
+             uint64_t desc0, uint64_t row0, uint64_t row_words, bool xcd_align) {
+        if (blocks_of_item == 0) return;
+        if (xcd_align) {
+            blocks = round8(blocks);
+            blocks_of_item = round8(blocks_of_item);
+        }
+        Seg s{};
+        s.block0 = blocks;
+        s.blocks = blocks_of_item;
+        s.nodes = nodes;
+        s.chunks = chunks;
+        s.desc0 = desc0;
+        s.row0 = row0;
+        s.row_words = row_words;
+        s.item = item;
+        segs.push_back(s);
+        blocks += blocks_of_item;
+    }
+    // Appends the segments to the batch table.
+    Launch commit(std::vector<Seg>& table) const {
+        Launch l;
+        l.first = static_cast<uint32_t>(table.size());
+        l.count = static_cast<uint32_t>(segs.size());
+        l.blocks = blocks;
+        table.insert(table.end(), segs.begin(), segs.end());
+        return l;
+    }
+};
+
+// Plans every launch of a batch, then queues them on ws.stream.  Results
+// (blob words) land in ws.blob at each job's word offset; per-item error flags
+// in ws.errs.
+// `out` (8-byte aligned, total_words * 8 bytes) receives the blobs, or null
+// for the workspace's own blob buffer.
+void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_t* out) {
     const hipStream_t st = ws.stream;
-    const uint64_t blob_words = ceil_div(plan.end_offset, 8);
-    ws.blob.ensure(blob_words * 8);
-    ws.plan.ensure(sizeof(Plan));
-    ws.err.ensure(sizeof(int));
-    BH_CHECK(hipMemcpyAsync(ws.plan.ptr, &plan, sizeof(Plan), hipMemcpyHostToDevice, st));
-    BH_CHECK(hipMemsetAsync(ws.err.ptr, 0, sizeof(int), st));
-    BH_CHECK(hipMemsetAsync(ws.blob.ptr, 0, blob_words * 8, st));
-    int* err = ws.err.as<int>();
-    const Plan* dplan = ws.plan.as<Plan>();
-    uint64_t* blob = ws.blob.as<uint64_t>();
-    const uint32_t df = std::min<uint32_t>(fusion_depth(n), plan.depth);
-    // Validation and the inverse: 32-bit while every index fits below the
-    // all-ones sentinel.
-    const bool wide_inverse = n >= 0xFFFFFFFFull;
-    const uint32_t blocks_n = static_cast<uint32_t>(ceil_div(n, kBlock));
-    ws.inv0.ensure(n * (wide_inverse ? 8 : 4));
-    BH_CHECK(hipMemsetAsync(ws.inv0.ptr, 0xFF, n * (wide_inverse ? 8 : 4), st));
+    const uint32_t count = static_cast<uint32_t>(jobs.size());
+    uint64_t total_n = 0, total_words = 0;
+    uint32_t max_df = 0;
+    bool wide_inverse = false;
+    for (Job& j : jobs) {
+        j.base = total_n;
+        j.blob = total_words;
+        j.df = fusion_depth(j.n, j.lay.plan.depth);
+        total_n += j.n;
+        total_words += ceil_div(j.lay.plan.end_offset, 8);
+        max_df = std::max(max_df, j.df);
+        wide_inverse = wide_inverse || j.n >= 0xFFFFFFFFull;
+    }
+    // Device copies of plans and item contexts.
+    ws.plans.ensure(sizeof(Plan) * count);
+    ws.items.ensure(sizeof(ItemCtx) * count);
+    ws.errs.ensure(sizeof(int) * count);
+    if (!out) ws.blob.ensure(total_words * 8);
+    uint8_t* blob_ptr = out ? out : ws.blob.as<uint8_t>();
+    std::vector<Plan> plans(count);
+    std::vector<ItemCtx> items(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        plans[i] = jobs[i].lay.plan;
+        items[i].src = jobs[i].dsrc;
+        items[i].plan = ws.plans.as<Plan>() + i;
+        items[i].n = jobs[i].n;
+        items[i].base = jobs[i].base;
+        items[i].blob = jobs[i].blob;
+        items[i].width = jobs[i].width;
+        items[i].df = jobs[i].df;
+        items[i].header_bytes = static_cast<uint32_t>(jobs[i].lay.header_bytes);
+        write_header(jobs[i].lay, items[i].header);
+    }
+
+    // Segment tables of every launch.
+    std::vector<Seg> table;
+    SegList per_entry_list;
+    for (uint32_t i = 0; i < count; ++i) {
+        per_entry_list.add(i, ceil_div(jobs[i].n, kBlock), 0, 0, 0, 0, 0, false);
+    }
+    const Launch per_entry = per_entry_list.commit(table);
+    struct LevelPlan {
+        Launch describe, fill, hbm, lds, pack[2];
+        bool wide_uf = false;
+    };
+    std::vector<LevelPlan> levels(max_df);
+    uint64_t max_desc = 1, max_rows = 1;
+    for (uint32_t d = 0; d < max_df; ++d) {
+        LevelPlan& lp = levels[d];
+        SegList describe, fill, hbm, lds, pack[2];
+        uint64_t desc_count = 0, row_count = 0;
+        const uint64_t nodes = uint64_t{1} << d;
+        for (uint32_t i = 0; i < count; ++i) {
+            if (jobs[i].df <= d) continue;
+            const uint64_t cmax = ceil_div(jobs[i].n, nodes);
+            const uint64_t maxpairs = cmax / 2;
+            const uint64_t row_words = std::max<uint64_t>(1, ceil_div(maxpairs, 64));
+            describe.add(i, ceil_div(nodes, 64), nodes, 0, desc_count, row_count, row_words,
+                         false);
+            if (cmax > kLevelLds) {
+                const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
+                hbm.add(i, nodes * chunks, nodes, chunks, desc_count, row_count, row_words, true);
+                fill.add(i, ceil_div(jobs[i].n, kBlock), 0, 0, 0, 0, 0, false);
+                lp.wide_uf = lp.wide_uf || maxpairs + 1 >= (uint64_t{1} << 31);
+            } else {
+                lds.add(i, nodes, nodes, 1, desc_count, row_count, row_words, false);
+            }
+            const Plan& P = jobs[i].lay.plan;
+            for (int side = 0; side < 2; ++side) {
+                const SideDepth& sd = P.side[side][d];
+                const uint32_t span_log2 = sd.group_depth - sd.relative_level - 1;
+                const uint64_t packets = ceil_div(maxpairs, uint64_t{1} << span_log2);
+                const uint64_t groups = uint64_t{1} << (d - sd.relative_level);
+                pack[side].add(i, ceil_div(groups * packets, kBlock / 64), nodes, packets,
+                               desc_count, row_count, row_words, false);
+            }
+            desc_count += nodes;
+            row_count += nodes * row_words;
+        }
+        lp.describe = describe.commit(table);
+        lp.fill = fill.commit(table);
+        lp.hbm = hbm.commit(table);
+        lp.lds = lds.commit(table);
+        lp.pack[0] = pack[0].commit(table);
+        lp.pack[1] = pack[1].commit(table);
+        max_desc = std::max(max_desc, desc_count);
+        max_rows = std::max(max_rows, row_count);
+    }
+    SegList subtree_list;
+    for (uint32_t i = 0; i < count; ++i) {
+        subtree_list.add(i, uint64_t{1} << jobs[i].df, 0, 0, 0, 0, 0, false);
+    }
+    const Launch subtree = subtree_list.commit(table);
+
+    // Scratch.
+    ws.segs.ensure(sizeof(Seg) * std::max<size_t>(1, table.size()));
+    ws.inv0.ensure(total_n * (wide_inverse ? 8 : 4));
+    if (max_df > 0) {
+        for (int i = 0; i < 2; ++i) {
+            ws.level[i].ensure(total_n * 4);
+            ws.ilevel[i].ensure(total_n * 4);
+        }
+        bool any_wide = false;
+        for (const LevelPlan& lp : levels) any_wide = any_wide || lp.wide_uf;
+        ws.uf.ensure(total_n * (any_wide ? 8 : 4));
+        ws.desc.ensure(sizeof(NodeDesc) * max_desc);
+        ws.srows.ensure(8 * max_rows);
+        ws.trows.ensure(8 * max_rows);
+    }
+    BH_CHECK(hipMemcpyAsync(ws.plans.ptr, plans.data(), sizeof(Plan) * count,
+                            hipMemcpyHostToDevice, st));
+    BH_CHECK(hipMemcpyAsync(ws.items.ptr, items.data(), sizeof(ItemCtx) * count,
+                            hipMemcpyHostToDevice, st));
+    BH_CHECK(hipMemcpyAsync(ws.segs.ptr, table.data(), sizeof(Seg) * table.size(),
+                            hipMemcpyHostToDevice, st));
+    BH_CHECK(hipMemsetAsync(ws.errs.ptr, 0, sizeof(int) * count, st));
+    BH_CHECK(hipMemsetAsync(blob_ptr, 0, total_words * 8, st));
+    BH_CHECK(hipMemsetAsync(ws.inv0.ptr, 0xFF, total_n * (wide_inverse ? 8 : 4), st));
+
+    const ItemCtx* ditems = ws.items.as<ItemCtx>();
+    const Seg* dsegs = ws.segs.as<Seg>();
+    int* errs = ws.errs.as<int>();
+    uint64_t* blob = reinterpret_cast<uint64_t*>(blob_ptr);
+    auto grid = [](const Launch& l) { return dim3(static_cast<uint32_t>(l.blocks)); };
+    auto segp = [&](const Launch& l) { return dsegs + l.first; };
+
     auto validate = [&](auto* inv) {
         using VI = std::remove_pointer_t<decltype(inv)>;
-        BH_LAUNCH(prof, "k_scatter_inverse", (k_scatter_inverse<V, VI>), dim3(blocks_n),
-                  dim3(kBlock), 0, st, src, n, inv, err);
-        BH_LAUNCH(prof, "k_check_inverse", (k_check_inverse<VI>), dim3(blocks_n), dim3(kBlock), 0,
-                  st, inv, n, err);
+        BH_LAUNCH(prof, "k_scatter_inverse", (k_scatter_inverse<VI>), grid(per_entry),
+                  dim3(kBlock), 0, st, ditems, segp(per_entry), per_entry.count, inv, errs);
+        BH_LAUNCH(prof, "k_check_inverse", (k_check_inverse<VI>), grid(per_entry), dim3(kBlock),
+                  0, st, ditems, segp(per_entry), per_entry.count, inv, errs);
     };
+    hipLaunchKernelGGL(k_headers, dim3(ceil_div(count, 64)), dim3(64), 0, st, ditems, count,
+                       blob_ptr);
     if (wide_inverse) validate(ws.inv0.as<uint64_t>());
     else validate(ws.inv0.as<uint32_t>());
-    if (df > 0) {
-        for (int i = 0; i < 2; ++i) {
-            ws.level[i].ensure(n * 4);
-            ws.ilevel[i].ensure(n * 4);
-        }
-    }
-    int next = 0;
-    for (uint32_t d = 0; d < df; ++d) {
-        const uint64_t nodes = uint64_t{1} << d;
-        const uint64_t cmax = ceil_div(n, nodes);
-        const uint64_t maxpairs = cmax / 2;
-        const uint64_t row_words = std::max<uint64_t>(1, ceil_div(maxpairs, 64));
-        ws.desc.ensure(nodes * sizeof(NodeDesc));
-        ws.srows.ensure(nodes * row_words * 8);
-        ws.trows.ensure(nodes * row_words * 8);
+
+    // Depth d reads level[(d - 1) & 1] and writes level[d & 1].
+    for (uint32_t d = 0; d < max_df; ++d) {
+        const LevelPlan& lp = levels[d];
         NodeDesc* desc = ws.desc.as<NodeDesc>();
-        BH_LAUNCH(prof, "k_describe", k_describe, dim3(static_cast<uint32_t>(ceil_div(nodes, 64))),
-                  dim3(64), 0, st, dplan, d, row_words, desc, err);
-        // Input slots of a node number ceil(c/2) < 2^31 for 32-bit words.
-        const bool wide = cmax / 2 + 1 >= (uint64_t{1} << 31);
-        ws.uf.ensure(n * (wide ? 8 : 4));
-        if (cmax > kLevelLds) BH_CHECK(hipMemsetAsync(ws.uf.ptr, 0xFF, n * (wide ? 8 : 4), st));
-        const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
-        const dim3 grid(static_cast<uint32_t>(chunks * nodes));
-        uint32_t* pdst = ws.level[next].as<uint32_t>();
-        uint32_t* idst = ws.ilevel[next].as<uint32_t>();
         uint64_t* srows = ws.srows.as<uint64_t>();
         uint64_t* trows = ws.trows.as<uint64_t>();
-        const bool in_lds = cmax <= kLevelLds;
-        auto run = [&](auto psrc, auto isrc, auto uf) {
-            using VS = std::remove_const_t<std::remove_pointer_t<decltype(psrc)>>;
-            using VI = std::remove_const_t<std::remove_pointer_t<decltype(isrc)>>;
+        const int next = d & 1;
+        const uint32_t* plev = ws.level[next ^ 1].as<uint32_t>();
+        uint32_t* pdst = ws.level[next].as<uint32_t>();
+        uint32_t* idst = ws.ilevel[next].as<uint32_t>();
+        const int level0 = d == 0 ? 1 : 0;
+        BH_LAUNCH(prof, "k_describe", k_describe, grid(lp.describe), dim3(64), 0, st, ditems,
+                  segp(lp.describe), lp.describe.count, d, desc, errs);
+        auto run = [&](auto* inv, auto* uf) {
+            using VI = std::remove_const_t<std::remove_pointer_t<decltype(inv)>>;
             using W = std::remove_pointer_t<decltype(uf)>;
-            if (in_lds) {
-                BH_LAUNCH(prof, "k_level_lds", (k_level_lds<VS, VI>),
-                          dim3(static_cast<uint32_t>(nodes)), dim3(kLevelThreads), 0, st, psrc,
-                          isrc, pdst, idst, desc, srows, trows, err);
-                return;
+            if (lp.hbm.count) {
+                BH_LAUNCH(prof, "k_fill_roots", (k_fill_roots<W>), grid(lp.fill), dim3(kBlock), 0,
+                          st, ditems, segp(lp.fill), lp.fill.count, uf);
+                BH_LAUNCH(prof, "k_union", (k_union<VI, W>), grid(lp.hbm), dim3(kBlock), 0, st,
+                          segp(lp.hbm), lp.hbm.count, inv, uf, desc, errs);
+                BH_LAUNCH(prof, "k_route_inputs", (k_route_inputs<W>), grid(lp.hbm), dim3(kBlock),
+                          0, st, ditems, segp(lp.hbm), lp.hbm.count, level0, plev, pdst, uf, desc,
+                          srows, errs);
+                BH_LAUNCH(prof, "k_route_outputs", (k_route_outputs<VI>), grid(lp.hbm),
+                          dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, idst, desc, srows,
+                          trows, errs);
             }
-            BH_LAUNCH(prof, "k_union", (k_union<VI, W>), grid, dim3(kBlock), 0, st, isrc, uf, desc,
-                      nodes, chunks, err);
-            BH_LAUNCH(prof, "k_route_inputs", (k_route_inputs<VS, W>), grid, dim3(kBlock), 0, st,
-                      psrc, pdst, uf, desc, nodes, chunks, srows, err);
-            BH_LAUNCH(prof, "k_route_outputs", (k_route_outputs<VI>), grid, dim3(kBlock), 0, st,
-                      isrc, idst, desc, nodes, chunks, srows, trows, err);
+            if (lp.lds.count) {
+                BH_LAUNCH(prof, "k_level_lds", (k_level_lds<VI>), grid(lp.lds),
+                          dim3(kLevelThreads), 0, st, ditems, segp(lp.lds), lp.lds.count, level0,
+                          plev, inv, pdst, idst, desc, srows, trows, errs);
+            }
         };
         if (d == 0) {
             if (wide_inverse) {
-                if (wide) run(src, ws.inv0.as<uint64_t>(), ws.uf.as<uint64_t>());
-                else run(src, ws.inv0.as<uint64_t>(), ws.uf.as<uint32_t>());
+                if (lp.wide_uf) run(ws.inv0.as<uint64_t>(), ws.uf.as<uint64_t>());
+                else run(ws.inv0.as<uint64_t>(), ws.uf.as<uint32_t>());
             } else {
-                if (wide) run(src, ws.inv0.as<uint32_t>(), ws.uf.as<uint64_t>());
-                else run(src, ws.inv0.as<uint32_t>(), ws.uf.as<uint32_t>());
+                if (lp.wide_uf) run(ws.inv0.as<uint32_t>(), ws.uf.as<uint64_t>());
+                else run(ws.inv0.as<uint32_t>(), ws.uf.as<uint32_t>());
             }
         } else {
-            const uint32_t* ps = ws.level[next ^ 1].as<uint32_t>();
-            const uint32_t* is = ws.ilevel[next ^ 1].as<uint32_t>();
-            if (wide) run(ps, is, ws.uf.as<uint64_t>());
-            else run(ps, is, ws.uf.as<uint32_t>());
+            const uint32_t* isrc = ws.ilevel[next ^ 1].as<uint32_t>();
+            if (lp.wide_uf) run(isrc, ws.uf.as<uint64_t>());
+            else run(isrc, ws.uf.as<uint32_t>());
         }
         for (int side = 0; side < 2; ++side) {
-            const SideDepth& sd = plan.side[side][d];
-            const uint32_t span_log2 = sd.group_depth - sd.relative_level - 1;
-            const uint64_t packets = ceil_div(maxpairs, uint64_t{1} << span_log2);
-            const uint64_t groups = uint64_t{1} << (d - sd.relative_level);
-            const uint64_t waves = groups * packets;
-            BH_LAUNCH(prof, "k_pack", k_pack,
-                      dim3(static_cast<uint32_t>(ceil_div(waves, kBlock / 64))), dim3(kBlock), 0,
-                      st, dplan, desc, side, d, packets, side == 0 ? srows : trows, blob, err);
+            BH_LAUNCH(prof, "k_pack", k_pack, grid(lp.pack[side]), dim3(kBlock), 0, st, ditems,
+                      segp(lp.pack[side]), lp.pack[side].count, side, d, desc,
+                      side == 0 ? srows : trows, blob, errs);
         }
-        next ^= 1;
     }
-    const uint64_t subtrees = uint64_t{1} << df;
-    if (df == 0) {
-        BH_LAUNCH(prof, "k_subtree", k_subtree<V>, dim3(static_cast<uint32_t>(subtrees)),
-                  dim3(kSubThreads), 0, st, src, dplan, df, blob, err);
-    } else {
-        BH_LAUNCH(prof, "k_subtree", k_subtree<uint32_t>, dim3(static_cast<uint32_t>(subtrees)),
-                  dim3(kSubThreads), 0, st, ws.level[next ^ 1].as<uint32_t>(), dplan, df, blob,
-                  err);
-    }
+    // An item with fusion depth df was last written at depth df - 1, into
+    // level[(df - 1) & 1]; items with df == 0 read their own source.
+    BH_LAUNCH(prof, "k_subtree", k_subtree, grid(subtree), dim3(kSubThreads), 0, st, ditems,
+              segp(subtree), subtree.count, ws.level[0].as<uint32_t>(),
+              ws.level[1].as<uint32_t>(), blob, errs);
     BH_CHECK(hipGetLastError());
 }
 
-void copy_err(Status& st, const char* msg, size_t len, char* err, size_t err_len) {
-    (void)len;
-    if (err && err_len) {
-        std::strncpy(err, msg, err_len - 1);
-        err[err_len - 1] = 0;
-    }
-    (void)st;
+uint64_t batch_words(const std::vector<Job>& jobs) {
+    uint64_t w = 0;
+    for (const Job& j : jobs) w += ceil_div(j.lay.plan.end_offset, 8);
+    return w;
 }
 
 int finish(const Status& st, char* err, size_t err_len) {
-    if (!st.ok()) copy_err(const_cast<Status&>(st), st.message.c_str(), st.message.size(), err,
-                           err_len);
+    if (!st.ok() && err && err_len) {
+        std::strncpy(err, st.message.c_str(), err_len - 1);
+        err[err_len - 1] = 0;
+    }
     return st.code;
 }
 
@@ -313,6 +459,19 @@ bool format_args(const benes_hip_format* f, FormatArgs& out, Status& st) {
     return true;
 }
 
+// Host-side checks of one request and its layout.
+bool prepare_job(const void* perm, uint64_t n, uint32_t width, const benes_hip_format* format,
+                 Job& job, Status& st) {
+    if (!check_input(n, width, st)) return false;
+    FormatArgs fa;
+    if (!format_args(format, fa, st)) return false;
+    if (!build_layout(n, fa, job.lay, st)) return false;
+    job.dsrc = perm;
+    job.n = n;
+    job.width = width;
+    return true;
+}
+
 int current_device(Status& st) {
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess) {
@@ -324,99 +483,132 @@ int current_device(Status& st) {
 
 const char* kNotPermutation = "input must contain every value from zero through size minus one";
 
-// Common body of both entry points.  `dst_host` (host API) or `dst_dev`
-// (device API) receives the blob.
-int compress_common(const void* perm, bool perm_on_device, uint64_t n, uint32_t width,
-                    const benes_hip_format* format, uint8_t* dst_host, uint8_t* dst_dev,
-                    uint64_t capacity, uint64_t* out_size, hipStream_t user_stream,
-                    double* device_seconds, char* err, size_t err_len) {
-    Status st;
-    if (!check_input(n, width, st)) return finish(st, err, err_len);
-    FormatArgs fa;
-    if (!format_args(format, fa, st)) return finish(st, err, err_len);
-    Layout lay;
-    if (!build_layout(n, fa, lay, st)) return finish(st, err, err_len);
-    if (out_size) *out_size = lay.plan.end_offset;
-    if ((!dst_host && !dst_dev) || capacity < lay.plan.end_offset) {
-        st.set(kBufferTooSmall, "output buffer is too small");
-        return finish(st, err, err_len);
+// Orders `after` behind the work already queued on `before`.
+void stream_after(hipStream_t after, hipStream_t before) {
+    hipEvent_t e;
+    BH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    BH_CHECK(hipEventRecord(e, before));
+    BH_CHECK(hipStreamWaitEvent(after, e, 0));
+    BH_CHECK(hipEventDestroy(e));
+}
+
+// Runs a batch whose inputs are on the device.  `out` receives the blobs at
+// their word offsets (null: the workspace blob).  Per-item error flags are
+// returned in `flags`.
+void run_batch(Workspace& ws, std::vector<Job>& jobs, uint8_t* out, hipStream_t user_stream,
+               std::vector<int>& flags, double* device_seconds) {
+    const hipStream_t s = ws.stream;
+    if (user_stream) stream_after(s, user_stream);
+    Profiler prof;
+    {
+        std::lock_guard<std::mutex> plock(g_prof_mu);
+        prof.on = g_profiling;
     }
+    prof.stream = s;
+    BH_CHECK(hipEventRecord(ws.ev_begin, s));
+    enqueue_batch(ws, jobs, prof, out);
+    BH_CHECK(hipEventRecord(ws.ev_end, s));
+    flags.assign(jobs.size(), 0);
+    BH_CHECK(hipMemcpyAsync(flags.data(), ws.errs.ptr, sizeof(int) * jobs.size(),
+                            hipMemcpyDeviceToHost, s));
+    BH_CHECK(hipStreamSynchronize(s));
+    prof.collect();
+    float ms = 0;
+    BH_CHECK(hipEventElapsedTime(&ms, ws.ev_begin, ws.ev_end));
+    g_device_seconds = ms / 1e3;
+    if (device_seconds) *device_seconds = g_device_seconds;
+    if (user_stream) stream_after(user_stream, s);
+}
+
+template <typename Body>
+int with_workspace(Status& st, Body&& body) {
     if (!benes_hip_available()) {
         st.set(kRuntime, "hip backend is unavailable");
-        return finish(st, err, err_len);
+        return st.code;
     }
     const int dev = current_device(st);
-    if (dev < 0) return finish(st, err, err_len);
+    if (dev < 0) return st.code;
     Workspace& ws = workspace_for(dev);
     std::lock_guard<std::mutex> lock(ws.mu);
     try {
         ws.init(dev);
-        const hipStream_t s = ws.stream;
-        if (user_stream) {
-            // Order after the caller's pending work on its stream.
-            hipEvent_t e;
-            BH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            BH_CHECK(hipEventRecord(e, user_stream));
-            BH_CHECK(hipStreamWaitEvent(s, e, 0));
-            BH_CHECK(hipEventDestroy(e));
-        }
-        const void* dsrc = perm;
-        if (!perm_on_device) {
-            ws.input.ensure(n * width);
-            BH_CHECK(hipMemcpyAsync(ws.input.ptr, perm, n * width, hipMemcpyHostToDevice, s));
-            dsrc = ws.input.ptr;
-        }
-        Profiler prof;
-        {
-            std::lock_guard<std::mutex> plock(g_prof_mu);
-            prof.on = g_profiling;
-        }
-        prof.stream = s;
-        BH_CHECK(hipEventRecord(ws.ev_begin, s));
-        if (width == 4) {
-            enqueue_pipeline(ws, static_cast<const uint32_t*>(dsrc), lay, prof);
-        } else {
-            enqueue_pipeline(ws, static_cast<const uint64_t*>(dsrc), lay, prof);
-        }
-        std::vector<uint8_t> header(lay.header_bytes);
-        write_header(lay, header.data());
-        if (dst_dev) {
-            BH_CHECK(hipMemcpyAsync(dst_dev, ws.blob.ptr, lay.plan.end_offset,
-                                    hipMemcpyDeviceToDevice, s));
-            BH_CHECK(hipMemcpyAsync(dst_dev, header.data(), header.size(), hipMemcpyHostToDevice,
-                                    s));
-        }
-        BH_CHECK(hipEventRecord(ws.ev_end, s));
-        int flag = 0;
-        BH_CHECK(hipMemcpyAsync(&flag, ws.err.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
-        if (dst_host) {
-            BH_CHECK(hipMemcpyAsync(dst_host, ws.blob.ptr, lay.plan.end_offset,
-                                    hipMemcpyDeviceToHost, s));
-        }
-        BH_CHECK(hipStreamSynchronize(s));
-        prof.collect();
-        float ms = 0;
-        BH_CHECK(hipEventElapsedTime(&ms, ws.ev_begin, ws.ev_end));
-        g_device_seconds = ms / 1e3;
-        if (device_seconds) *device_seconds = g_device_seconds;
-        if (user_stream) {
-            hipEvent_t e;
-            BH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            BH_CHECK(hipEventRecord(e, s));
-            BH_CHECK(hipStreamWaitEvent(user_stream, e, 0));
-            BH_CHECK(hipEventDestroy(e));
-        }
-        if (flag) {
-            st.set(kInvalidArgument, kNotPermutation);
-            return finish(st, err, err_len);
-        }
-        if (dst_host) std::memcpy(dst_host, header.data(), header.size());
+        body(ws);
     } catch (const HipError& e) {
         st.set(kRuntime, std::string("hip error: ") + hipGetErrorString(e.code) + " in " + e.what);
     } catch (const std::bad_alloc&) {
         st.set(kLength, "allocation failed");
     }
+    return st.code;
+}
+
+int compress_one(const void* perm, bool on_device, uint64_t n, uint32_t width,
+                 const benes_hip_format* format, uint8_t* dst_host, uint8_t* dst_dev,
+                 uint64_t capacity, uint64_t* out_size, hipStream_t user_stream,
+                 double* device_seconds, char* err, size_t err_len) {
+    Status st;
+    std::vector<Job> jobs(1);
+    if (!prepare_job(perm, n, width, format, jobs[0], st)) return finish(st, err, err_len);
+    const uint64_t end = jobs[0].lay.plan.end_offset;
+    if (out_size) *out_size = end;
+    if ((!dst_host && !dst_dev) || capacity < end) {
+        st.set(kBufferTooSmall, "output buffer is too small");
+        return finish(st, err, err_len);
+    }
+    with_workspace(st, [&](Workspace& ws) {
+        if (!on_device) {
+            ws.input.ensure(n * width);
+            BH_CHECK(hipMemcpyAsync(ws.input.ptr, perm, n * width, hipMemcpyHostToDevice,
+                                    ws.stream));
+            jobs[0].dsrc = ws.input.ptr;
+        }
+        // Write straight into the caller's buffer when it can take whole words.
+        const bool direct = dst_dev && (reinterpret_cast<uintptr_t>(dst_dev) & 7) == 0 &&
+                            capacity >= round8(end);
+        std::vector<int> flags;
+        run_batch(ws, jobs, direct ? dst_dev : nullptr, user_stream, flags, device_seconds);
+        if (flags[0]) {
+            st.set(kInvalidArgument, kNotPermutation);
+            return;
+        }
+        if (dst_dev && !direct) {
+            BH_CHECK(hipMemcpyAsync(dst_dev, ws.blob.ptr, end, hipMemcpyDeviceToDevice,
+                                    ws.stream));
+            BH_CHECK(hipStreamSynchronize(ws.stream));
+        }
+        if (dst_host) {
+            BH_CHECK(hipMemcpy(dst_host, ws.blob.ptr, end, hipMemcpyDeviceToHost));
+        }
+    });
     return finish(st, err, err_len);
+}
+
+// Prepares the jobs of a batch; items failing host checks get their status
+// and are left out (`index` maps jobs back to items).
+void prepare_batch(benes_hip_item* items, uint64_t count, std::vector<Job>& jobs,
+                   std::vector<uint64_t>& index) {
+    for (uint64_t i = 0; i < count; ++i) {
+        Status st;
+        Job job;
+        items[i].out_size = 0;
+        items[i].status = BENES_HIP_OK;
+        items[i].message[0] = 0;
+        if (!prepare_job(items[i].perm, items[i].n, items[i].width_bytes, &items[i].format, job,
+                         st)) {
+            items[i].status = st.code;
+            std::strncpy(items[i].message, st.message.c_str(), sizeof(items[i].message) - 1);
+            items[i].message[sizeof(items[i].message) - 1] = 0;
+            continue;
+        }
+        items[i].out_size = job.lay.plan.end_offset;
+        jobs.push_back(std::move(job));
+        index.push_back(i);
+    }
+}
+
+void mark_invalid(benes_hip_item& item) {
+    item.status = BENES_HIP_INVALID_ARGUMENT;
+    std::strncpy(item.message, kNotPermutation, sizeof(item.message) - 1);
+    item.message[sizeof(item.message) - 1] = 0;
 }
 
 }  // namespace
@@ -448,16 +640,94 @@ int benes_hip_available(void) {
 int benes_hip_compress(const void* perm, uint64_t n, uint32_t width_bytes,
                        const benes_hip_format* format, uint8_t* out, uint64_t out_capacity,
                        uint64_t* out_size, double* device_seconds, char* err, size_t err_len) {
-    return compress_common(perm, false, n, width_bytes, format, out, nullptr, out_capacity,
-                           out_size, nullptr, device_seconds, err, err_len);
+    return compress_one(perm, false, n, width_bytes, format, out, nullptr, out_capacity,
+                        out_size, nullptr, device_seconds, err, err_len);
 }
 
 int benes_hip_compress_device(const void* d_perm, uint64_t n, uint32_t width_bytes,
                               const benes_hip_format* format, uint8_t* d_out,
                               uint64_t out_capacity, uint64_t* out_size, void* stream, char* err,
                               size_t err_len) {
-    return compress_common(d_perm, true, n, width_bytes, format, nullptr, d_out, out_capacity,
-                           out_size, static_cast<hipStream_t>(stream), nullptr, err, err_len);
+    return compress_one(d_perm, true, n, width_bytes, format, nullptr, d_out, out_capacity,
+                        out_size, static_cast<hipStream_t>(stream), nullptr, err, err_len);
+}
+
+int benes_hip_batch_layout(benes_hip_item* items, uint64_t count, uint64_t* total_bytes) {
+    std::vector<Job> jobs;
+    std::vector<uint64_t> index;
+    prepare_batch(items, count, jobs, index);
+    uint64_t words = 0;
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        items[index[k]].out_offset = words * 8;
+        words += ceil_div(jobs[k].lay.plan.end_offset, 8);
+    }
+    if (total_bytes) *total_bytes = words * 8;
+    return BENES_HIP_OK;
+}
+
+int benes_hip_compress_batch(benes_hip_item* items, uint64_t count, double* device_seconds,
+                             char* err, size_t err_len) {
+    Status st;
+    std::vector<Job> jobs;
+    std::vector<uint64_t> index;
+    prepare_batch(items, count, jobs, index);
+    if (jobs.empty()) return finish(st, err, err_len);
+    with_workspace(st, [&](Workspace& ws) {
+        uint64_t bytes = 0;
+        std::vector<uint64_t> at(jobs.size());
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            at[k] = bytes;
+            bytes += round8(jobs[k].n * jobs[k].width);
+        }
+        ws.input.ensure(bytes);
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            BH_CHECK(hipMemcpyAsync(ws.input.as<uint8_t>() + at[k], jobs[k].dsrc,
+                                    jobs[k].n * jobs[k].width, hipMemcpyHostToDevice, ws.stream));
+            jobs[k].dsrc = ws.input.as<uint8_t>() + at[k];
+        }
+        std::vector<int> flags;
+        run_batch(ws, jobs, nullptr, nullptr, flags, device_seconds);
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            benes_hip_item& item = items[index[k]];
+            if (flags[k]) {
+                mark_invalid(item);
+                continue;
+            }
+            if (!item.out || item.out_capacity < item.out_size) {
+                item.status = BENES_HIP_BUFFER_TOO_SMALL;
+                continue;
+            }
+            BH_CHECK(hipMemcpyAsync(item.out, ws.blob.as<uint8_t>() + jobs[k].blob * 8,
+                                    item.out_size, hipMemcpyDeviceToHost, ws.stream));
+        }
+        BH_CHECK(hipStreamSynchronize(ws.stream));
+    });
+    return finish(st, err, err_len);
+}
+
+int benes_hip_compress_batch_device(benes_hip_item* items, uint64_t count, uint8_t* d_out,
+                                    uint64_t out_capacity, void* stream, double* device_seconds,
+                                    char* err, size_t err_len) {
+    Status st;
+    std::vector<Job> jobs;
+    std::vector<uint64_t> index;
+    prepare_batch(items, count, jobs, index);
+    if (jobs.empty()) return finish(st, err, err_len);
+    const uint64_t need = batch_words(jobs) * 8;
+    if (!d_out || out_capacity < need || (reinterpret_cast<uintptr_t>(d_out) & 7) != 0) {
+        st.set(kBufferTooSmall, "batch output buffer is too small or not 8-byte aligned");
+        return finish(st, err, err_len);
+    }
+    with_workspace(st, [&](Workspace& ws) {
+        std::vector<int> flags;
+        run_batch(ws, jobs, d_out, static_cast<hipStream_t>(stream), flags, device_seconds);
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            benes_hip_item& item = items[index[k]];
+            item.out_offset = jobs[k].blob * 8;
+            if (flags[k]) mark_invalid(item);
+        }
+    });
+    return finish(st, err, err_len);
 }
 
 int benes_hip_encoded_size(uint64_t n, const benes_hip_format* format, uint64_t* out_size,

@@ -7,6 +7,12 @@
 // Levels ping-pong between two u32 arrays, so routing is a pure streaming
 // pass; only the union-find touches memory at random.
 //
+// Every launch serves a batch: item i's entries live at [base_i, base_i + n_i)
+// of the shared arrays and its blob at word blob_i of the shared blob.  A
+// launch carries a table of segments (one per item taking part) giving the
+// item's first block; a block finds its segment by binary search.  A plain
+// compress is a batch of one.
+//
 // Before the first level:
 //   k_scatter_inverse  validation and the inverse permutation, no atomics
 // Per "global" level (nodes larger than one workgroup's LDS):
@@ -37,11 +43,55 @@ constexpr uint32_t kSub = 1u << kSubLog2;
 constexpr uint32_t kSubThreads = 1024;
 
 struct NodeDesc {
-    uint64_t off;
+    uint64_t off;  // absolute offset of the node's entries in the shared arrays
     uint64_t cnt;
     uint64_t row;  // first word of the node's bit rows
     SideAddr a[2];
 };
+
+// One permutation of a batch.
+struct ItemCtx {
+    const void* src;    // its entries (device), width bytes each
+    const Plan* plan;   // its format tables (device)
+    uint64_t n;
+    uint64_t base;      // offset of its entries in the shared arrays
+    uint64_t blob;      // offset of its blob in the shared blob, in words
+    uint32_t width;     // 4 or 8
+    uint32_t df;        // fusion depth: first depth handled by k_subtree
+    uint32_t header_bytes;
+    uint8_t header[152];  // 20 fixed bytes + both cluster lists (at most 64 each)
+};
+
+// One item's share of a launch.
+struct Seg {
+    uint64_t block0;     // first block of the item in the launch
+    uint64_t blocks;     // blocks of the item (a multiple of 8 when XCD-mapped)
+    uint64_t nodes;      // nodes of the item at this depth
+    uint64_t chunks;     // blocks per node, or packets per group (k_pack)
+    uint64_t desc0;      // index of the item's first NodeDesc at this depth
+    uint64_t row0;       // first row word of the item at this depth
+    uint64_t row_words;  // row words per node
+    uint32_t item;
+    uint32_t pad;
+};
+
+// Segment of the block: the last one whose first block is <= blockIdx.x.
+__device__ __forceinline__ const Seg& find_seg(const Seg* segs, uint32_t nsegs) {
+    const uint64_t b = blockIdx.x;
+    uint32_t lo = 0, hi = nsegs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (segs[mid].block0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return segs[lo];
+}
+
+// Entry i of an item's level-0 permutation.
+__device__ __forceinline__ uint64_t load_src(const ItemCtx& it, uint64_t i) {
+    return it.width == 4 ? static_cast<const uint32_t*>(it.src)[i]
+                         : static_cast<const uint64_t*>(it.src)[i];
+}
 
 // ---------------------------------------------------------------------------
 // Wave-level helpers (wave64)
@@ -102,23 +152,60 @@ __device__ __forceinline__ uint32_t row_bit(const uint64_t* rows, uint64_t row, 
 // Input validation and the inverse (benes.cpp:152-173).
 //
 // inv starts as all-ones; every in-range entry stores its index at its value.
-// A value out of range raises the error flag; a duplicate leaves some other
-// slot unwritten, which the first union pass (it reads every inverse pair)
-// reports.  No atomics: on this device every global atomic costs ~4x a random
-// store.
+// A value out of range raises the item's error flag; a duplicate leaves some
+// other slot unwritten, which k_check_inverse reports.  No atomics: on this
+// device every global atomic costs several random stores.
 
-template <typename V, typename VI>
-__global__ __launch_bounds__(kBlock) void k_scatter_inverse(const V* __restrict__ perm,
-                                                            uint64_t n, VI* __restrict__ inv,
-                                                            int* __restrict__ err) {
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t v = static_cast<uint64_t>(perm[i]);
-    if (v >= n) {
-        *err = 1;
+template <typename VI>
+__global__ __launch_bounds__(kBlock) void k_scatter_inverse(const ItemCtx* __restrict__ items,
+                                                            const Seg* __restrict__ segs,
+                                                            uint32_t nsegs, VI* __restrict__ inv,
+                                                            int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    const ItemCtx it = items[sg.item];
+    const uint64_t i = (blockIdx.x - sg.block0) * kBlock + threadIdx.x;
+    if (i >= it.n) return;
+    const uint64_t v = load_src(it, i);
+    if (v >= it.n) {
+        errs[sg.item] = 1;
         return;
     }
-    inv[v] = static_cast<VI>(i);
+    inv[it.base + v] = static_cast<VI>(i);
+}
+
+// A slot of the inverse still all-ones was reached by no entry: some value
+// occurred twice.
+template <typename VI>
+__global__ __launch_bounds__(kBlock) void k_check_inverse(const ItemCtx* __restrict__ items,
+                                                          const Seg* __restrict__ segs,
+                                                          uint32_t nsegs, const VI* __restrict__ inv,
+                                                          int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    const ItemCtx it = items[sg.item];
+    const uint64_t i = (blockIdx.x - sg.block0) * kBlock + threadIdx.x;
+    if (i < it.n && inv[it.base + i] == static_cast<VI>(~VI{0})) errs[sg.item] = 1;
+}
+
+// Sets words [base, base + n) of each item to all-ones (union-find roots).
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_fill_roots(const ItemCtx* __restrict__ items,
+                                                       const Seg* __restrict__ segs,
+                                                       uint32_t nsegs, W* __restrict__ uf) {
+    const Seg& sg = find_seg(segs, nsegs);
+    const ItemCtx it = items[sg.item];
+    const uint64_t i = (blockIdx.x - sg.block0) * kBlock + threadIdx.x;
+    if (i < it.n) uf[it.base + i] = UfWord<W>::root;
+}
+
+// Writes each item's header bytes (internal.hpp:431-454) at the start of its
+// zeroed blob, before any switch bit is OR'ed in.
+__global__ void k_headers(const ItemCtx* __restrict__ items, uint32_t count,
+                          uint8_t* __restrict__ blob_all) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const ItemCtx& it = items[i];
+    uint8_t* dst = blob_all + it.blob * 8;
+    for (uint32_t b = 0; b < it.header_bytes; ++b) dst[b] = it.header[b];
 }
 
 // ---------------------------------------------------------------------------
@@ -133,40 +220,43 @@ __global__ __launch_bounds__(kBlock) void k_scatter_inverse(const V* __restrict_
 // left P[p] = P[2p + s_p] >> 1 and left I[q] = I[2q + t_q] >> 1.
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
-// When a level has at least 8 nodes, blocks are renumbered so that every
-// block of node j lands on XCD j % 8: the node's union-find words, arrays and
-// rows then live in that one XCD's L2.
+// When an item has at least 8 nodes at a depth (its segment then starts on a
+// multiple of 8), blocks are renumbered so that every block of node j lands
+// on XCD j % 8, the node's union-find words, arrays and rows then living in
+// that one XCD's L2; consecutive blocks of one XCD visit its nodes in turn,
+// so the blocks in flight spread over all of them instead of crowding one
+// node's hottest words.
 constexpr uint32_t kXcds = 8;
 
-__device__ __forceinline__ void block_to_node(uint64_t nodes, uint64_t chunks, uint64_t& node,
-                                              uint64_t& chunk) {
-    const uint64_t b = blockIdx.x;
-    if (nodes % kXcds != 0) {
-        node = b / chunks;
-        chunk = b % chunks;
-        return;
+__device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uint64_t& chunk) {
+    const uint64_t b = blockIdx.x - sg.block0;
+    if (sg.nodes % kXcds != 0) {
+        node = b / sg.chunks;
+        chunk = b % sg.chunks;
+    } else {
+        const uint64_t xcd = b % kXcds, local = b / kXcds, per_xcd = sg.nodes / kXcds;
+        node = (local % per_xcd) * kXcds + xcd;
+        chunk = local / per_xcd;
     }
-    // Consecutive blocks of one XCD visit its nodes in turn, so the blocks in
-    // flight on an XCD spread over all of its nodes instead of crowding one
-    // node's hottest union-find words.
-    const uint64_t xcd = b % kXcds, local = b / kXcds, per_xcd = nodes / kXcds;
-    node = (local % per_xcd) * kXcds + xcd;
-    chunk = local / per_xcd;
+    return node < sg.nodes && chunk < sg.chunks;
 }
 
-__global__ void k_describe(const Plan* __restrict__ plan, uint32_t d, uint64_t row_words,
-                           NodeDesc* __restrict__ desc, const int* __restrict__ err) {
-    if (*err) return;
-    const Plan& P = *plan;
-    const uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= (uint64_t{1} << d)) return;
+__global__ void k_describe(const ItemCtx* __restrict__ items, const Seg* __restrict__ segs,
+                           uint32_t nsegs, uint32_t d, NodeDesc* __restrict__ desc,
+                           const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    const ItemCtx it = items[sg.item];
+    const Plan& P = *it.plan;
+    const uint64_t j = (blockIdx.x - sg.block0) * blockDim.x + threadIdx.x;
+    if (j >= sg.nodes) return;
     NodeDesc nd;
-    nd.off = node_offset(P.n, d, j);
+    nd.off = it.base + node_offset(P.n, d, j);
     nd.cnt = node_count(P.n, d, j);
-    nd.row = j * row_words;
+    nd.row = sg.row0 + j * sg.row_words;
     nd.a[0] = node_address(P, 0, d, j);
     nd.a[1] = node_address(P, 1, d, j);
-    desc[j] = nd;
+    desc[sg.desc0 + j] = nd;
 }
 
 // Input slot of entry x of a node of `count` entries: the odd tail is slot 0.
@@ -175,26 +265,18 @@ __device__ __forceinline__ uint64_t in_slot(uint64_t x, uint64_t count) {
     return x >> 1;
 }
 
-// A slot of the inverse still all-ones was reached by no entry: some value
-// occurred twice.
-template <typename VI>
-__global__ __launch_bounds__(kBlock) void k_check_inverse(const VI* __restrict__ inv, uint64_t n,
-                                                          int* __restrict__ err) {
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n && inv[i] == static_cast<VI>(~VI{0})) *err = 1;
-}
-
 // One thread per output pair: union of the two input slots that reach it.
 template <typename VI, typename W>
-__global__ __launch_bounds__(kBlock) void k_union(const VI* __restrict__ inv,
-                                                  W* __restrict__ uf,
+__global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, uint32_t nsegs,
+                                                  const VI* __restrict__ inv, W* __restrict__ uf,
                                                   const NodeDesc* __restrict__ desc,
-                                                  uint64_t nodes, uint64_t chunks,
-                                                  const int* __restrict__ err) {
-    if (*err) return;
+                                                  const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
     uint64_t node, chunk;
-    block_to_node(nodes, chunks, node, chunk);
-    const uint64_t off = desc[node].off, cnt = desc[node].cnt;
+    if (!block_to_node(sg, node, chunk)) return;
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, cnt = nd.cnt;
     const uint64_t q = chunk * kBlock + threadIdx.x;
     if (q >= (cnt >> 1)) return;
     const uint64_t a = static_cast<uint64_t>(inv[off + 2 * q]);
@@ -205,31 +287,43 @@ __global__ __launch_bounds__(kBlock) void k_union(const VI* __restrict__ inv,
                                  static_cast<uint32_t>(1 ^ ((a ^ b) & 1)));
 }
 
+// Entry i (node-local index `off + i` in shared terms) of the level's
+// permutation: the item's own source at depth 0, the shared level array below.
+__device__ __forceinline__ uint64_t load_perm(const ItemCtx& it, const uint32_t* lev, bool level0,
+                                              uint64_t abs) {
+    return level0 ? load_src(it, abs - it.base) : static_cast<uint64_t>(lev[abs]);
+}
+
 // One thread per input pair: canonical switch value s_p (its slot's parity to
 // the root), child routing of P, and s ballot-packed into the node's row.
-template <typename V, typename W>
-__global__ __launch_bounds__(kBlock) void k_route_inputs(const V* __restrict__ src,
-                                                         uint32_t* __restrict__ dst,
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restrict__ items,
+                                                         const Seg* __restrict__ segs,
+                                                         uint32_t nsegs, int level0,
+                                                         const uint32_t* __restrict__ plev,
+                                                         uint32_t* __restrict__ pdst,
                                                          W* __restrict__ uf,
                                                          const NodeDesc* __restrict__ desc,
-                                                         uint64_t nodes, uint64_t chunks,
                                                          uint64_t* __restrict__ srows,
-                                                         const int* __restrict__ err) {
-    if (*err) return;
+                                                         const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
     uint64_t node, chunk;
-    block_to_node(nodes, chunks, node, chunk);
-    const uint64_t off = desc[node].off, cnt = desc[node].cnt, row = desc[node].row;
+    if (!block_to_node(sg, node, chunk)) return;
+    const ItemCtx it = items[sg.item];
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, cnt = nd.cnt, row = nd.row;
     const uint64_t h = cnt >> 1, lc = h + (cnt & 1);
     const uint64_t p = chunk * kBlock + threadIdx.x;
     uint32_t s = 0;
     if (p < h) {
         uf_find<DeviceForest>(uf + off, static_cast<W>(p + (cnt & 1)), s);
-        const uint64_t a0 = static_cast<uint64_t>(src[off + 2 * p]);
-        const uint64_t a1 = static_cast<uint64_t>(src[off + 2 * p + 1]);
-        dst[off + p] = static_cast<uint32_t>((s ? a1 : a0) >> 1);
-        dst[off + lc + p] = static_cast<uint32_t>((s ? a0 : a1) >> 1);
+        const uint64_t a0 = load_perm(it, plev, level0, off + 2 * p);
+        const uint64_t a1 = load_perm(it, plev, level0, off + 2 * p + 1);
+        pdst[off + p] = static_cast<uint32_t>((s ? a1 : a0) >> 1);
+        pdst[off + lc + p] = static_cast<uint32_t>((s ? a0 : a1) >> 1);
     } else if ((cnt & 1) && p == h) {
-        dst[off + h] = static_cast<uint32_t>(static_cast<uint64_t>(src[off + cnt - 1]) >> 1);
+        pdst[off + h] = static_cast<uint32_t>(load_perm(it, plev, level0, off + cnt - 1) >> 1);
     }
     const uint64_t ms = __ballot(s != 0);
     const uint64_t first = p - lane_id();
@@ -239,17 +333,20 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const V* __restrict__ s
 // One thread per output pair: t_q = colour(I[2q]) read from the input rows,
 // child routing of I, and t ballot-packed into the node's row.
 template <typename VI>
-__global__ __launch_bounds__(kBlock) void k_route_outputs(const VI* __restrict__ inv,
-                                                          uint32_t* __restrict__ dst,
+__global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict__ segs,
+                                                          uint32_t nsegs,
+                                                          const VI* __restrict__ inv,
+                                                          uint32_t* __restrict__ idst,
                                                           const NodeDesc* __restrict__ desc,
-                                                          uint64_t nodes, uint64_t chunks,
                                                           const uint64_t* __restrict__ srows,
                                                           uint64_t* __restrict__ trows,
-                                                          const int* __restrict__ err) {
-    if (*err) return;
+                                                          const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
     uint64_t node, chunk;
-    block_to_node(nodes, chunks, node, chunk);
-    const uint64_t off = desc[node].off, cnt = desc[node].cnt, row = desc[node].row;
+    if (!block_to_node(sg, node, chunk)) return;
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, cnt = nd.cnt, row = nd.row;
     const uint64_t h = cnt >> 1, lc = h + (cnt & 1);
     const uint64_t q = chunk * kBlock + threadIdx.x;
     uint32_t t = 0;
@@ -258,10 +355,10 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const VI* __restrict__
         const uint64_t a1 = static_cast<uint64_t>(inv[off + 2 * q + 1]);
         // colour(x) = s_{x>>1} ^ (x & 1); the odd tail entry has colour 0.
         if (!((cnt & 1) && a0 == cnt - 1)) t = row_bit(srows, row, a0 >> 1) ^ (a0 & 1);
-        dst[off + q] = static_cast<uint32_t>((t ? a1 : a0) >> 1);
-        dst[off + lc + q] = static_cast<uint32_t>((t ? a0 : a1) >> 1);
+        idst[off + q] = static_cast<uint32_t>((t ? a1 : a0) >> 1);
+        idst[off + lc + q] = static_cast<uint32_t>((t ? a0 : a1) >> 1);
     } else if ((cnt & 1) && q == h) {
-        dst[off + h] = static_cast<uint32_t>(static_cast<uint64_t>(inv[off + cnt - 1]) >> 1);
+        idst[off + h] = static_cast<uint32_t>(static_cast<uint64_t>(inv[off + cnt - 1]) >> 1);
     }
     const uint64_t mt = __ballot(t != 0);
     const uint64_t first = q - lane_id();
@@ -276,17 +373,21 @@ constexpr uint32_t kLevelLdsLog2 = 16;
 constexpr uint32_t kLevelLds = 1u << kLevelLdsLog2;
 constexpr uint32_t kLevelThreads = 1024;
 
-template <typename V, typename VI>
+template <typename VI>
 __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
-    const V* __restrict__ psrc, const VI* __restrict__ isrc, uint32_t* __restrict__ pdst,
+    const ItemCtx* __restrict__ items, const Seg* __restrict__ segs, uint32_t nsegs, int level0,
+    const uint32_t* __restrict__ plev, const VI* __restrict__ isrc, uint32_t* __restrict__ pdst,
     uint32_t* __restrict__ idst, const NodeDesc* __restrict__ desc, uint64_t* __restrict__ srows,
-    uint64_t* __restrict__ trows, const int* __restrict__ err) {
-    if (*err) return;
+    uint64_t* __restrict__ trows, const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
     __shared__ uint32_t uf[kLevelLds / 2 + 1];
     __shared__ uint64_t sbits[kLevelLds / 128 + 1];
-    const uint64_t node = blockIdx.x;
-    const uint64_t off = desc[node].off, row = desc[node].row;
-    const uint32_t cnt = static_cast<uint32_t>(desc[node].cnt);
+    const ItemCtx it = items[sg.item];
+    const uint64_t node = blockIdx.x - sg.block0;
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, row = nd.row;
+    const uint32_t cnt = static_cast<uint32_t>(nd.cnt);
     const uint32_t h = cnt >> 1, odd = cnt & 1, lc = h + odd;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nthreads = blockDim.x, nwaves = blockDim.x >> 6;
@@ -306,12 +407,12 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
         uint32_t s = 0;
         if (p < h) {
             uf_find<GroupForest>(uf, p + odd, s);
-            const uint64_t a0 = static_cast<uint64_t>(psrc[off + 2 * p]);
-            const uint64_t a1 = static_cast<uint64_t>(psrc[off + 2 * p + 1]);
+            const uint64_t a0 = load_perm(it, plev, level0, off + 2 * p);
+            const uint64_t a1 = load_perm(it, plev, level0, off + 2 * p + 1);
             pdst[off + p] = static_cast<uint32_t>((s ? a1 : a0) >> 1);
             pdst[off + lc + p] = static_cast<uint32_t>((s ? a0 : a1) >> 1);
         } else if (odd && p == h) {
-            pdst[off + h] = static_cast<uint32_t>(static_cast<uint64_t>(psrc[off + cnt - 1]) >> 1);
+            pdst[off + h] = static_cast<uint32_t>(load_perm(it, plev, level0, off + cnt - 1) >> 1);
         }
         const uint64_t ms = __ballot(s != 0);
         if (lane == 0) {
@@ -341,16 +442,22 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
     }
 }
 
-// One wave per (group, packet) at depth d on one side.
-__global__ __launch_bounds__(kBlock) void k_pack(const Plan* __restrict__ plan,
-                                                 const NodeDesc* __restrict__ desc, int side,
-                                                 uint32_t d, uint64_t packets,
+// One wave per (cluster group, packet) of one item at depth d on one side.
+__global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ items,
+                                                 const Seg* __restrict__ segs, uint32_t nsegs,
+                                                 int side, uint32_t d,
+                                                 const NodeDesc* __restrict__ desc,
                                                  const uint64_t* __restrict__ rows,
-                                                 uint64_t* __restrict__ blob,
-                                                 const int* __restrict__ err) {
-    if (*err) return;
-    const Plan& P = *plan;
-    const uint64_t gw = static_cast<uint64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+                                                 uint64_t* __restrict__ blob_all,
+                                                 const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    const ItemCtx it = items[sg.item];
+    const Plan& P = *it.plan;
+    uint64_t* blob = blob_all + it.blob;
+    const NodeDesc* dsc = desc + sg.desc0;
+    const uint64_t packets = sg.chunks;
+    const uint64_t gw = (blockIdx.x - sg.block0) * (kBlock / 64) + (threadIdx.x >> 6);
     const SideDepth sd = P.side[side][d];
     const uint32_t r = sd.relative_level, g = sd.group_depth;
     const uint32_t root_depth = d - r;
@@ -365,17 +472,17 @@ __global__ __launch_bounds__(kBlock) void k_pack(const Plan* __restrict__ plan,
     const uint64_t secbase = (side == 0 ? P.input_offset : P.output_offset) * 8;
     if (min_pairs >= first_pair + (uint64_t{1} << span_log2)) {
         // Every node of the level fills its share of this packet: one run.
-        const uint64_t start = bit_of(desc[b0].a[side], first_pair);
+        const uint64_t start = bit_of(dsc[b0].a[side], first_pair);
         const uint64_t len = uint64_t{1} << (r + span_log2);
         const uint64_t mask = (uint64_t{1} << span_log2) - 1;
         wave_write_run(blob, secbase + start, len, [&](uint64_t k) {
-            const NodeDesc& nd = desc[b0 + (k >> span_log2)];
+            const NodeDesc& nd = dsc[b0 + (k >> span_log2)];
             return row_bit(rows, nd.row, first_pair + (k & mask)) != 0;
         });
     } else {
         // The last packet: node shares differ, place each from its own address.
         for (uint64_t pi = 0; pi < (uint64_t{1} << r); ++pi) {
-            const NodeDesc& nd = desc[b0 + pi];
+            const NodeDesc& nd = dsc[b0 + pi];
             const uint64_t len = packet_share(nd.cnt, packet, span_log2, P.leaf);
             if (len == 0) continue;
             const uint64_t start = bit_of(nd.a[side], first_pair);
@@ -500,16 +607,27 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
         const uint32_t nsets = one_set ? 1u : (1u << (R - d0));
         const uint32_t set_nodes = one_set ? (1u << k) : (1u << r);
         const uint32_t packets = static_cast<uint32_t>((maxpairs + span - 1) >> span_log2);
-        for (uint32_t job = wave; job < nsets * packets; job += nwaves) {
+        SideAddr cached{};
+        uint32_t cached_set = ~0u;
+        // Each wave takes a contiguous range of jobs, so it revisits one set.
+        const uint32_t jobs = nsets * packets;
+        const uint32_t per_wave = (jobs + nwaves - 1) / nwaves;
+        const uint32_t job_end = min(jobs, (wave + 1) * per_wave);
+        for (uint32_t job = wave * per_wave; job < job_end; ++job) {
             const uint32_t set = job / packets, pk = job - set * packets;
             const uint32_t first = set * set_nodes;
             const uint32_t minc = one_set ? (n0 >> k) : (sub_count(n0, R - d0, set) >> r);
             const uint64_t first_pair = static_cast<uint64_t>(pk) << span_log2;
             if (minc > L && (minc >> 1) >= first_pair + span) {
-                // Every node fills its share of this packet: one run.
-                const uint64_t path = (b0 << k) | first;
-                const SideAddr a = address_at(P, walk_from(P, side, root.walk[side], d0, d, path),
-                                              d, path);
+                // Every node fills its share of this packet: one run.  The
+                // set's first node's address serves every packet of the set.
+                if (set != cached_set) {
+                    const uint64_t path = (b0 << k) | first;
+                    cached = address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d,
+                                        path);
+                    cached_set = set;
+                }
+                const SideAddr& a = cached;
                 wave_write_run(blob, secbase + bit_of(a, first_pair),
                                static_cast<uint64_t>(set_nodes) << span_log2, [&](uint64_t q) {
                                    const uint32_t pos =
@@ -537,21 +655,27 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
     }
 }
 
-template <typename V>
-__global__ __launch_bounds__(kSubThreads) void k_subtree(const V* __restrict__ src,
-                                                         const Plan* __restrict__ plan,
-                                                         uint32_t d0, uint64_t* __restrict__ blob,
-                                                         const int* __restrict__ err) {
-    if (*err) return;
+__global__ __launch_bounds__(kSubThreads) void k_subtree(const ItemCtx* __restrict__ items,
+                                                         const Seg* __restrict__ segs,
+                                                         uint32_t nsegs,
+                                                         const uint32_t* __restrict__ lev0,
+                                                         const uint32_t* __restrict__ lev1,
+                                                         uint64_t* __restrict__ blob_all,
+                                                         const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
     __shared__ uint16_t buf[2][kSub];
     __shared__ uint32_t uf[kSub];
     __shared__ uint64_t sbits[kSub / 64 + 2];
     __shared__ uint64_t tbits[kSub / 64 + 2];
     __shared__ uint16_t noff[kSub / 2 + 1];
     __shared__ SubtreeRoot root;
-    const Plan& P = *plan;
+    const ItemCtx it = items[sg.item];
+    const Plan& P = *it.plan;
+    uint64_t* blob = blob_all + it.blob;
     const uint32_t L = P.leaf;
-    const uint64_t b0 = blockIdx.x;
+    const uint32_t d0 = it.df;
+    const uint64_t b0 = blockIdx.x - sg.block0;
     const uint32_t n0 = static_cast<uint32_t>(node_count(P.n, d0, b0));
     const uint64_t base = node_offset(P.n, d0, b0);
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -561,7 +685,12 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const V* __restrict__ s
         root.walk[1] = walk_to(P, 1, d0, b0);
         root.middle = middle_from(P, 0, 0, d0, b0);
     }
-    for (uint32_t i = tid; i < n0; i += nthreads) buf[0][i] = static_cast<uint16_t>(src[base + i]);
+    // Depth d0 - 1 wrote level (d0 - 1) & 1; depth 0 reads the item's source.
+    const uint32_t* plev = ((d0 - 1) & 1) ? lev1 : lev0;
+    for (uint32_t i = tid; i < n0; i += nthreads) {
+        buf[0][i] = static_cast<uint16_t>(d0 == 0 ? load_src(it, base + i)
+                                                  : plev[it.base + base + i]);
+    }
     __syncthreads();
     const int rank_side = P.placement == 1 ? 0 : 1;
     const uint64_t rank_section = leaf_section_offset(P) * 8;

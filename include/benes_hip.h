@@ -75,6 +75,39 @@ int benes_hip_compress_device(const void* d_perm, uint64_t n, uint32_t width_byt
                               uint8_t* d_out, uint64_t out_capacity, uint64_t* out_size,
                               void* stream, char* err, size_t err_len);
 
+/* ---- Batches: many permutations of uneven sizes and widths in one call. ----
+ * Every launch of the pipeline covers all items of the batch at once; the
+ * bytes of each blob are the same as those of a plain compress of the item.
+ * Items failing validation get their own status and message; the others
+ * still complete.  No counterpart in the reference, whose compress() takes one
+ * permutation per call. */
+typedef struct benes_hip_item {
+    const void* perm;         /* host (compress_batch) or device (_device) pointer */
+    uint64_t n;
+    uint32_t width_bytes;     /* 4 or 8 */
+    int32_t status;           /* [out] BENES_HIP_* of this item */
+    benes_hip_format format;
+    uint8_t* out;             /* compress_batch: host destination of the blob */
+    uint64_t out_capacity;    /* compress_batch: its capacity in bytes */
+    uint64_t out_offset;      /* [out] _device / batch_layout: byte offset in d_out */
+    uint64_t out_size;        /* [out] blob bytes */
+    char message[96];         /* [out] error text when status != OK */
+} benes_hip_item;
+
+/* Host-only: validates formats, fills out_size and out_offset of every item
+ * and the device bytes a compress_batch_device output needs (*total_bytes). */
+int benes_hip_batch_layout(benes_hip_item* items, uint64_t count, uint64_t* total_bytes);
+
+/* Host inputs and outputs (one staging copy in, one copy out per item). */
+int benes_hip_compress_batch(benes_hip_item* items, uint64_t count, double* device_seconds,
+                             char* err, size_t err_len);
+
+/* Device inputs; every blob is written into d_out (8-byte aligned, at least
+ * benes_hip_batch_layout's total) at its item's out_offset. */
+int benes_hip_compress_batch_device(benes_hip_item* items, uint64_t count, uint8_t* d_out,
+                                    uint64_t out_capacity, void* stream, double* device_seconds,
+                                    char* err, size_t err_len);
+
 /* Size of the blob for n entries (reference make_layout, src/internal.hpp:389-429). */
 int benes_hip_encoded_size(uint64_t n, const benes_hip_format* format,
                            uint64_t* out_size, char* err, size_t err_len);
