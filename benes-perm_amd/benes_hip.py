@@ -25,7 +25,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbenes_hip.so")
 
-OK, INVALID_ARGUMENT, OVERFLOW, LENGTH, LOGIC, RUNTIME, BUFFER_TOO_SMALL = range(7)
+OK, INVALID_ARGUMENT, OVERFLOW, LENGTH, LOGIC, RUNTIME, BUFFER_TOO_SMALL, PENDING = range(8)
 
 SEPARATE, EMBEDDED_INPUT, EMBEDDED_OUTPUT = 0, 1, 2
 FISHER_YATES, SINGLE_CYCLE, NEAR_IDENTITY, IDENTITY, REVERSE = range(5)
@@ -38,6 +38,11 @@ EXPORTS = (
     "benes_hip_batch_layout",
     "benes_hip_compress_batch",
     "benes_hip_compress_batch_device",
+    "benes_hip_server_create",
+    "benes_hip_server_submit",
+    "benes_hip_server_wait",
+    "benes_hip_server_stats",
+    "benes_hip_server_destroy",
     "benes_hip_encoded_size",
     "benes_hip_last_device_seconds",
     "benes_hip_set_profiling",
@@ -73,6 +78,12 @@ class _Item(ctypes.Structure):
         ("out_size", ctypes.c_uint64),
         ("message", ctypes.c_char * 96),
     ]
+
+
+class _Counters(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "completed", "entries", "batches", "peak_bytes", "max_concurrent_batches", "queued",
+        "budget_bytes")]
 
 
 class _Layout(ctypes.Structure):
@@ -135,6 +146,16 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_compress_batch_device.argtypes = [ip, u64, vp, u64, vp,
                                                     ctypes.POINTER(ctypes.c_double), cp, sz]
     lib.benes_hip_last_device_seconds.restype = ctypes.c_double
+    lib.benes_hip_server_create.argtypes = [u64, u32, u32, cp, sz]
+    lib.benes_hip_server_create.restype = vp
+    lib.benes_hip_server_submit.argtypes = [vp, vp, u64, u32, fp, vp, u64,
+                                            ctypes.POINTER(ctypes.c_int), cp, sz]
+    lib.benes_hip_server_submit.restype = u64
+    lib.benes_hip_server_wait.argtypes = [vp, u64, ctypes.c_int64, ctypes.POINTER(u64),
+                                          ctypes.POINTER(ctypes.c_double), cp, sz]
+    lib.benes_hip_server_stats.argtypes = [vp, ctypes.POINTER(_Counters)]
+    lib.benes_hip_server_destroy.argtypes = [vp]
+    lib.benes_hip_server_destroy.restype = None
     lib.benes_hip_set_profiling.argtypes = [ctypes.c_int]
     lib.benes_hip_set_profiling.restype = None
     lib.benes_hip_kernel_stats.argtypes = [ctypes.c_int, cp, sz, ctypes.POINTER(ctypes.c_double),
@@ -290,6 +311,65 @@ def compress_batch_device(perm_ptrs, ns, widths, fmts, d_out_ptr, capacity, stre
                                                      ctypes.byref(secs), err, 256), err)
     return ([it.out_offset for it in arr], [it.out_size for it in arr],
             [_item_error(it) for it in arr], secs.value)
+
+
+class Server:
+    """The request loop of include/benes_hip.h (continuous batching over
+    `lanes` device streams, admission against an HBM budget)."""
+
+    def __init__(self, budget_bytes: int, max_batch: int = 512, lanes: int = 2):
+        err = ctypes.create_string_buffer(256)
+        self._h = library().benes_hip_server_create(budget_bytes, max_batch, lanes, err, 256)
+        if not self._h:
+            raise RuntimeError(err.value.decode())
+        self._pending = {}
+
+    def submit(self, perm: np.ndarray, fmt: FormatOptions | None = None) -> int:
+        perm = np.ascontiguousarray(perm)
+        f, keep = _format(fmt)
+        size = encoded_size(perm.size, fmt)
+        out = np.zeros(size, np.uint8)
+        status = ctypes.c_int()
+        err = ctypes.create_string_buffer(256)
+        rid = library().benes_hip_server_submit(self._h, perm.ctypes.data, perm.size,
+                                                perm.dtype.itemsize, ctypes.byref(f),
+                                                out.ctypes.data, out.size, ctypes.byref(status),
+                                                err, 256)
+        if rid == 0:
+            _raise(status.value, err)
+        self._pending[rid] = (perm, out, keep)
+        return rid
+
+    def wait(self, rid: int, timeout_ms: int = -1):
+        """(blob bytes or the exception of the request, latency seconds);
+        raises TimeoutError, keeping the request, when timeout_ms runs out."""
+        size = ctypes.c_uint64()
+        lat = ctypes.c_double()
+        err = ctypes.create_string_buffer(256)
+        rc = library().benes_hip_server_wait(self._h, rid, timeout_ms, ctypes.byref(size),
+                                             ctypes.byref(lat), err, 256)
+        if rc == PENDING:
+            raise TimeoutError(f"request {rid} still running")
+        perm, out, keep = self._pending.pop(rid)
+        if rc != OK:
+            return _ERRORS.get(rc, RuntimeError)(err.value.decode()), lat.value
+        return out[:size.value].tobytes(), lat.value
+
+    def stats(self) -> dict:
+        c = _Counters()
+        library().benes_hip_server_stats(self._h, ctypes.byref(c))
+        return {name: getattr(c, name) for name, _ in _Counters._fields_}
+
+    def close(self):
+        if self._h:
+            library().benes_hip_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def last_device_seconds() -> float:

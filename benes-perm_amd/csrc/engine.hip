@@ -14,7 +14,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
 #include <cstdio>
+#include <deque>
+#include <thread>
+#include <unordered_map>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -611,7 +616,222 @@ void mark_invalid(benes_hip_item& item) {
     item.message[sizeof(item.message) - 1] = 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Request loop (continuous batching)
+//
+// Requests queue in arrival order.  A scheduler thread owns `lanes` streams,
+// each with its own workspace (the routing state of its batch stays in HBM).
+// Whenever a lane is idle it takes the longest prefix of the queue that fits
+// the lane's share of the HBM budget and the batch cap, copies the inputs to
+// pinned staging, and queues the whole batch on its stream without waiting;
+// requests that arrive meanwhile are picked up by the next free lane, so
+// batches on different lanes overlap on the device.  A finished batch copies
+// each blob to its request's buffer and wakes its waiters.
+
+struct Request {
+    uint64_t id = 0;
+    Job job;
+    const void* host_perm = nullptr;
+    uint8_t* out = nullptr;
+    uint64_t out_capacity = 0;
+    uint64_t footprint = 0;
+    std::chrono::steady_clock::time_point submitted, done;
+    int status = -1;  // -1 pending
+    std::string message;
+};
+
+struct Lane {
+    Workspace ws;
+    std::vector<Request*> batch;
+    std::vector<Job> jobs;
+    uint8_t* pinned_in = nullptr;
+    size_t pinned_in_bytes = 0;
+    uint8_t* pinned_out = nullptr;
+    size_t pinned_out_bytes = 0;
+    int* pinned_flags = nullptr;
+    size_t pinned_flags_count = 0;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    uint64_t held = 0;  // budget bytes of the batch in flight
+};
+
+uint64_t request_footprint(uint64_t n, uint32_t width, uint64_t blob_bytes) {
+    return n * (width + 28) + n / 4 + round8(blob_bytes) + (64u << 10);
+}
+
+void ensure_pinned(uint8_t*& p, size_t& have, size_t need) {
+    if (need <= have) return;
+    if (p) BH_CHECK(hipHostFree(p));
+    p = nullptr;
+    have = 0;
+    BH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), need, hipHostMallocDefault));
+    have = need;
+}
+
 }  // namespace
+
+struct Server {
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::unordered_map<uint64_t, Request*> requests;
+    std::vector<Lane*> lanes;
+    std::thread thread;
+    bool stop = false;
+    int device = 0;
+    uint64_t budget = 0, lane_cap = 0, max_batch = 0, next_id = 1;
+    uint64_t in_flight_bytes = 0, peak_bytes = 0, batches = 0, completed = 0, entries = 0;
+    uint64_t max_concurrent = 0;
+    std::string fatal;
+
+    void run();
+    bool launch(Lane& lane);
+    void complete(Lane& lane);
+};
+
+namespace {
+
+}  // namespace
+
+bool Server::launch(Lane& lane) {
+    // Caller holds mu.  Take the queue prefix that fits this lane.
+    uint64_t bytes = 0, in_bytes = 0, out_bytes = 0;
+    lane.batch.clear();
+    while (!queue.empty() && lane.batch.size() < max_batch) {
+        Request* r = queue.front();
+        if (bytes + r->footprint > lane_cap || in_flight_bytes + bytes + r->footprint > budget) {
+            break;
+        }
+        bytes += r->footprint;
+        in_bytes += round8(r->job.n * r->job.width);
+        out_bytes += round8(r->job.lay.plan.end_offset);
+        lane.batch.push_back(r);
+        queue.pop_front();
+    }
+    if (lane.batch.empty()) return false;
+    lane.held = bytes;
+    in_flight_bytes += bytes;
+    peak_bytes = std::max(peak_bytes, in_flight_bytes);
+    ++batches;
+    lane.busy = true;
+    std::vector<Request*> batch = lane.batch;
+    mu.unlock();
+    try {
+        BH_CHECK(hipSetDevice(device));
+        lane.ws.init(device);
+        ensure_pinned(lane.pinned_in, lane.pinned_in_bytes, in_bytes);
+        ensure_pinned(lane.pinned_out, lane.pinned_out_bytes, out_bytes);
+        if (lane.pinned_flags_count < batch.size()) {
+            if (lane.pinned_flags) BH_CHECK(hipHostFree(lane.pinned_flags));
+            BH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&lane.pinned_flags),
+                                   sizeof(int) * batch.size(), hipHostMallocDefault));
+            lane.pinned_flags_count = batch.size();
+        }
+        lane.ws.input.ensure(in_bytes);
+        lane.jobs.clear();
+        uint64_t at = 0;
+        for (Request* r : batch) {
+            const uint64_t len = r->job.n * r->job.width;
+            std::memcpy(lane.pinned_in + at, r->host_perm, len);
+            Job j = r->job;
+            j.dsrc = lane.ws.input.as<uint8_t>() + at;
+            lane.jobs.push_back(std::move(j));
+            at += round8(len);
+        }
+        const hipStream_t s = lane.ws.stream;
+        BH_CHECK(hipMemcpyAsync(lane.ws.input.ptr, lane.pinned_in, at, hipMemcpyHostToDevice, s));
+        Profiler prof;
+        prof.stream = s;
+        enqueue_batch(lane.ws, lane.jobs, prof, nullptr);
+        BH_CHECK(hipMemcpyAsync(lane.pinned_out, lane.ws.blob.ptr, out_bytes,
+                                hipMemcpyDeviceToHost, s));
+        BH_CHECK(hipMemcpyAsync(lane.pinned_flags, lane.ws.errs.ptr, sizeof(int) * batch.size(),
+                                hipMemcpyDeviceToHost, s));
+        if (!lane.done) BH_CHECK(hipEventCreateWithFlags(&lane.done, hipEventDisableTiming));
+        BH_CHECK(hipEventRecord(lane.done, s));
+    } catch (const HipError& e) {
+        mu.lock();
+        fatal = std::string("hip error: ") + hipGetErrorString(e.code) + " in " + e.what;
+        return true;
+    } catch (const std::bad_alloc&) {
+        mu.lock();
+        fatal = "allocation failed";
+        return true;
+    }
+    mu.lock();
+    return true;
+}
+
+void Server::complete(Lane& lane) {
+    // Caller holds mu; the lane's event has completed.
+    const auto now = std::chrono::steady_clock::now();
+    for (size_t k = 0; k < lane.batch.size(); ++k) {
+        Request* r = lane.batch[k];
+        if (lane.pinned_flags[k]) {
+            r->status = kInvalidArgument;
+            r->message = kNotPermutation;
+        } else if (!r->out || r->out_capacity < r->job.lay.plan.end_offset) {
+            r->status = kBufferTooSmall;
+            r->message = "output buffer is too small";
+        } else {
+            std::memcpy(r->out, lane.pinned_out + lane.jobs[k].blob * 8,
+                        r->job.lay.plan.end_offset);
+            r->status = kOk;
+            entries += r->job.n;
+        }
+        r->done = now;
+        ++completed;
+    }
+    lane.busy = false;
+    in_flight_bytes -= lane.held;
+    lane.held = 0;
+    lane.batch.clear();
+    cv_done.notify_all();
+}
+
+void Server::run() {
+    std::unique_lock<std::mutex> lock(mu);
+    while (true) {
+        if (stop && queue.empty() &&
+            std::none_of(lanes.begin(), lanes.end(), [](Lane* l) { return l->busy; })) {
+            break;
+        }
+        bool progressed = false;
+        for (Lane* lane : lanes) {
+            if (lane->busy && hipEventQuery(lane->done) == hipSuccess) {
+                complete(*lane);
+                progressed = true;
+            }
+        }
+        for (Lane* lane : lanes) {
+            if (!lane->busy && !queue.empty() && fatal.empty()) {
+                if (launch(*lane)) progressed = true;
+            }
+        }
+        uint64_t busy = 0;
+        for (Lane* lane : lanes) busy += lane->busy ? 1 : 0;
+        max_concurrent = std::max(max_concurrent, busy);
+        if (!fatal.empty()) {
+            // Fail everything still waiting; in-flight lanes finish normally.
+            for (Request* r : queue) {
+                r->status = kRuntime;
+                r->message = fatal;
+                r->done = std::chrono::steady_clock::now();
+            }
+            queue.clear();
+            cv_done.notify_all();
+        }
+        if (!progressed) {
+            if (busy) {
+                cv_work.wait_for(lock, std::chrono::microseconds(50));
+            } else {
+                cv_work.wait(lock, [&] { return stop || !queue.empty(); });
+            }
+        }
+    }
+}
+
 }  // namespace benes_hip
 
 using namespace benes_hip;
@@ -827,4 +1047,121 @@ uint64_t benes_hip_address_bit(const benes_hip_address* a, uint64_t pair) {
            pair % a->pair_span;
 }
 
+struct benes_hip_server : benes_hip::Server {};
+
+benes_hip_server* benes_hip_server_create(uint64_t hbm_budget_bytes, uint32_t max_batch,
+                                          uint32_t lanes, char* err, size_t err_len) {
+    Status st;
+    if (!benes_hip_available()) {
+        st.set(kRuntime, "hip backend is unavailable");
+        finish(st, err, err_len);
+        return nullptr;
+    }
+    const int dev = current_device(st);
+    if (dev < 0) {
+        finish(st, err, err_len);
+        return nullptr;
+    }
+    auto* srv = new benes_hip_server();
+    srv->device = dev;
+    srv->budget = hbm_budget_bytes;
+    lanes = std::max<uint32_t>(1, lanes);
+    srv->lane_cap = hbm_budget_bytes / lanes;
+    srv->max_batch = std::max<uint32_t>(1, max_batch);
+    for (uint32_t i = 0; i < lanes; ++i) srv->lanes.push_back(new Lane());
+    srv->thread = std::thread([srv] { srv->run(); });
+    return srv;
+}
+
+uint64_t benes_hip_server_submit(benes_hip_server* srv, const void* perm, uint64_t n,
+                                 uint32_t width_bytes, const benes_hip_format* format,
+                                 uint8_t* out, uint64_t out_capacity, int* status, char* err,
+                                 size_t err_len) {
+    Status st;
+    auto* r = new Request();
+    if (!prepare_job(perm, n, width_bytes, format, r->job, st)) {
+        delete r;
+        if (status) *status = finish(st, err, err_len);
+        return 0;
+    }
+    r->host_perm = perm;
+    r->out = out;
+    r->out_capacity = out_capacity;
+    r->footprint = request_footprint(n, width_bytes, r->job.lay.plan.end_offset);
+    r->submitted = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lock(srv->mu);
+    if (r->footprint > srv->lane_cap) {
+        delete r;
+        st.set(kLength, "request exceeds the server's HBM budget");
+        if (status) *status = finish(st, err, err_len);
+        return 0;
+    }
+    r->id = srv->next_id++;
+    srv->requests[r->id] = r;
+    srv->queue.push_back(r);
+    srv->cv_work.notify_one();
+    if (status) *status = BENES_HIP_OK;
+    return r->id;
+}
+
+int benes_hip_server_wait(benes_hip_server* srv, uint64_t id, int64_t timeout_ms,
+                          uint64_t* out_size, double* latency_seconds, char* err,
+                          size_t err_len) {
+    Status st;
+    std::unique_lock<std::mutex> lock(srv->mu);
+    auto it = srv->requests.find(id);
+    if (it == srv->requests.end()) {
+        st.set(kInvalidArgument, "unknown request id");
+        return finish(st, err, err_len);
+    }
+    Request* r = it->second;
+    auto ready = [&] { return r->status != -1; };
+    if (timeout_ms < 0) {
+        srv->cv_done.wait(lock, ready);
+    } else if (!srv->cv_done.wait_for(lock, std::chrono::milliseconds(timeout_ms), ready)) {
+        st.set(kPending, "request still running");
+        return finish(st, err, err_len);
+    }
+    if (out_size) *out_size = r->job.lay.plan.end_offset;
+    if (latency_seconds) {
+        *latency_seconds = std::chrono::duration<double>(r->done - r->submitted).count();
+    }
+    st.set(r->status, r->message);
+    srv->requests.erase(it);
+    delete r;
+    return finish(st, err, err_len);
+}
+
+int benes_hip_server_stats(benes_hip_server* srv, benes_hip_server_counters* out) {
+    std::lock_guard<std::mutex> lock(srv->mu);
+    out->completed = srv->completed;
+    out->entries = srv->entries;
+    out->batches = srv->batches;
+    out->peak_bytes = srv->peak_bytes;
+    out->max_concurrent_batches = srv->max_concurrent;
+    out->queued = srv->queue.size();
+    out->budget_bytes = srv->budget;
+    return BENES_HIP_OK;
+}
+
+void benes_hip_server_destroy(benes_hip_server* srv) {
+    if (!srv) return;
+    {
+        std::lock_guard<std::mutex> lock(srv->mu);
+        srv->stop = true;
+        srv->cv_work.notify_all();
+    }
+    srv->thread.join();
+    for (auto& kv : srv->requests) delete kv.second;
+    for (Lane* lane : srv->lanes) {
+        if (lane->pinned_in) (void)hipHostFree(lane->pinned_in);
+        if (lane->pinned_out) (void)hipHostFree(lane->pinned_out);
+        if (lane->pinned_flags) (void)hipHostFree(lane->pinned_flags);
+        if (lane->done) (void)hipEventDestroy(lane->done);
+        delete lane;
+    }
+    delete srv;
+}
+
 }  // extern "C"
+

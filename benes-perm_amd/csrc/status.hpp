@@ -19,6 +19,7 @@ enum Code : int {
     kLogic = BENES_HIP_LOGIC,
     kRuntime = BENES_HIP_RUNTIME,
     kBufferTooSmall = BENES_HIP_BUFFER_TOO_SMALL,
+    kPending = BENES_HIP_PENDING,
 };
 
 struct Status {

@@ -38,6 +38,7 @@ extern "C" {
 #define BENES_HIP_LOGIC 4            /* std::logic_error      */
 #define BENES_HIP_RUNTIME 5          /* std::runtime_error (device error, unavailable) */
 #define BENES_HIP_BUFFER_TOO_SMALL 6 /* out_capacity < needed; *out_size holds the need */
+#define BENES_HIP_PENDING 7          /* benes_hip_server_wait timed out; request still running */
 
 /* FormatOptions (reference include/benes/benes.hpp:32-39).  Empty cluster
  * lists (count 0) select the balanced default, as in the reference. */
@@ -107,6 +108,45 @@ int benes_hip_compress_batch(benes_hip_item* items, uint64_t count, double* devi
 int benes_hip_compress_batch_device(benes_hip_item* items, uint64_t count, uint8_t* d_out,
                                     uint64_t out_capacity, void* stream, double* device_seconds,
                                     char* err, size_t err_len);
+
+/* ---- Serving: a request loop with continuous batching. ----
+ * No counterpart in the reference.  Requests (host arrays, any size and width,
+ * any format) are queued on submit; a scheduler thread batches the queue's
+ * oldest requests whenever one of `lanes` device streams is idle, admitting
+ * only what fits that lane's share of hbm_budget_bytes, so batches on
+ * different lanes overlap on the device and requests arriving meanwhile join
+ * the next batch.  Each blob is identical to a plain compress of its request.
+ * The caller keeps `perm` readable and `out` writable until the request
+ * completes (benes_hip_server_wait).  A request larger than one lane's share
+ * of the budget is refused with BENES_HIP_LENGTH. */
+typedef struct benes_hip_server benes_hip_server;
+
+typedef struct benes_hip_server_counters {
+    uint64_t completed;              /* requests finished (any status) */
+    uint64_t entries;                /* entries of requests finished OK */
+    uint64_t batches;                /* batches launched */
+    uint64_t peak_bytes;             /* largest admitted footprint in flight */
+    uint64_t max_concurrent_batches; /* most lanes busy at once */
+    uint64_t queued;                 /* requests waiting now */
+    uint64_t budget_bytes;
+} benes_hip_server_counters;
+
+benes_hip_server* benes_hip_server_create(uint64_t hbm_budget_bytes, uint32_t max_batch,
+                                          uint32_t lanes, char* err, size_t err_len);
+/* Returns the request id (>= 1), or 0 with *status set when refused. */
+uint64_t benes_hip_server_submit(benes_hip_server* server, const void* perm, uint64_t n,
+                                 uint32_t width_bytes, const benes_hip_format* format,
+                                 uint8_t* out, uint64_t out_capacity, int* status, char* err,
+                                 size_t err_len);
+/* Waits (timeout_ms < 0: without limit) and returns the request's status;
+ * *latency_seconds is the time from submit to the blob being in `out`.
+ * BENES_HIP_PENDING means the time ran out first; any other status releases
+ * the id. */
+int benes_hip_server_wait(benes_hip_server* server, uint64_t id, int64_t timeout_ms,
+                          uint64_t* out_size, double* latency_seconds, char* err, size_t err_len);
+int benes_hip_server_stats(benes_hip_server* server, benes_hip_server_counters* out);
+/* Finishes queued and running requests, then stops the loop. */
+void benes_hip_server_destroy(benes_hip_server* server);
 
 /* Size of the blob for n entries (reference make_layout, src/internal.hpp:389-429). */
 int benes_hip_encoded_size(uint64_t n, const benes_hip_format* format,
