@@ -52,22 +52,25 @@ def global_levels(n: int, sub: int = 1 << 14) -> int:
     return d
 
 
-def algorithmic_bytes(kernel: str, n: int, blob: int, levels: int) -> float:
-    """Bytes one launch of `kernel` must move at minimum (per-entry figures
-    times the entries one launch covers)."""
-    if kernel == "k_validate":
-        return 4.0 * n                    # read the source once
-    if kernel == "k_union":
-        return 4.0 * n                    # read the level's permutation
-    if kernel == "k_resolve":
-        return 8.0 * n                    # read it again, write the children
+def algorithmic_bytes(kernel: str, n: int, blob: int) -> float:
+    """Bytes one launch of `kernel` must move at minimum: the per-entry
+    figures times the entries one launch covers (one launch of a level
+    kernel covers one depth of the whole permutation)."""
+    per_launch = {
+        "k_union": 6.0 * n,             # inverse read (4 B/entry), slot parent words written (4 B/slot)
+        "k_route_inputs": 10.0 * n + n / 8,   # P read, children written, slot words read, row bits
+        "k_route_outputs": 8.0 * n + n / 4,   # I read, children written, row bits
+        "k_level_lds": 16.0 * n + n / 4,      # P and I read, both children written, row bits
+        "k_scatter_inverse": 8.0 * n,   # source read, inverse written
+        "k_check_inverse": 4.0 * n,
+        "k_fill_roots": 4.0 * n,
+        "k_pack": n / 8.0,              # one side's row bits of one depth, read and written
+    }
     if kernel == "k_subtree":
         lay = B.layout_of(n, FMT)
-        global_switch = levels * (n // 2) * 2 / 8.0
-        return 4.0 * n + (blob - lay["input_offset"] - global_switch)  # read + its blob share
-    if kernel == "k_pack":
-        return 2.0 * (n // 2) / 8.0       # read one side's bit rows, write them
-    return 0.0
+        switch_bits_above = global_levels(n) * (n // 2) * 2 / 8.0
+        return 4.0 * n + (blob - lay["input_offset"] - switch_bits_above)
+    return per_launch.get(kernel, 0.0)
 
 
 def cpu_baseline(perm: np.ndarray, want: str | None):
@@ -92,12 +95,26 @@ def cpu_baseline(perm: np.ndarray, want: str | None):
             "sample": "2^22 entries, single-threaded oracle", "seconds": dt}
 
 
+def calibrate_copy(nbytes: int = 256 << 20, reps: int = 5):
+    """A copy of known size, to scale the FETCH_SIZE / WRITE_SIZE counters:
+    each launch reads and writes exactly `nbytes`."""
+    src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read_per_launch": nbytes,
+                      "bytes_written_per_launch": nbytes, "launches": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this backend compresses one permutation (or one "
@@ -105,6 +122,9 @@ def main():
         sys.exit(2)
 
     torch.cuda.init()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     assert B.available(), "gfx950 backend unavailable"
     t0 = time.perf_counter()
     perm = B.generate(B.FISHER_YATES, N, SEED)
@@ -138,18 +158,21 @@ def main():
         step()
     stats = B.kernel_stats()
     B.set_profiling(False)
-    levels = global_levels(N)
     total_ms = sum(ms for ms, _ in stats.values())
     dom = max(stats, key=lambda k: stats[k][0])
     dom_ms, dom_launches = stats[dom]
     avg_ms = dom_ms / dom_launches
-    alg = algorithmic_bytes(dom, N, size, levels)
+    alg = algorithmic_bytes(dom, N, size)
     achieved = alg / (avg_ms * 1e-3) / 1e9
+    # HBM bytes per launch of the same kernel from the committed counter runs
+    # (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy's reading).
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(dom)
+            rec = json.load(f).get("kernels", {}).get(dom)
+        if rec:
+            traffic = rec["bytes_per_launch"]
 
     ms_per_step = elapsed / args.steps * 1e3
     line = {
