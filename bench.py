@@ -37,12 +37,17 @@ FMT = B.FormatOptions(5, (11, 10), (10, 11), B.EMBEDDED_INPUT)
 HBM_PEAK_GBS = 8000.0
 
 
-def fixture_digest():
+def fixture(name: str):
     with open(os.path.join(ROOT, "tests", "golden", "fixtures.json")) as f:
         for rec in json.load(f):
-            if rec["name"] == "cfg2-fy-2^26":
-                return rec["sha256"]
+            if rec["name"] == name:
+                return rec
     return None
+
+
+def fixture_digest(name: str = "cfg2-fy-2^26"):
+    rec = fixture(name)
+    return rec["sha256"] if rec else None
 
 
 def global_levels(n: int, sub: int = 1 << 14) -> int:
@@ -107,12 +112,221 @@ def calibrate_copy(nbytes: int = 256 << 20, reps: int = 5):
                       "bytes_written_per_launch": nbytes, "launches": reps}), flush=True)
 
 
+def other_inputs(d_out, size, stream, steps: int):
+    """The single N-cycle (Sattolo) and identity-with-1%-transpositions inputs
+    of the same 2^26 workload, each byte-checked against its digest."""
+    out = {}
+    for name, label in (("cfg2-cycle-2^26", "single_cycle"), ("cfg2-nearid-2^26", "near_identity_1pct")):
+        rec = fixture(name)
+        perm = B.generate(rec["kind"], rec["n"], rec["seed"], rec["rate"])
+        d_perm = torch.from_numpy(perm.view(np.int32)).cuda()
+        del perm
+
+        def step():
+            B.compress_device(d_perm.data_ptr(), N, 4, d_out.data_ptr(), size, FMT, stream)
+
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        digest = hashlib.sha256(d_out.cpu().numpy().tobytes()).hexdigest()
+        out[label] = {"seed": rec["seed"], "rate_ppm": rec["rate"], "ms_per_step": dt * 1e3,
+                      "Mentries_per_s": N / dt / 1e6, "byte_exact": digest == rec["sha256"]}
+        del d_perm
+    return out
+
+
+def host_inclusive(perm: np.ndarray, want: str | None, reps: int = 3):
+    """benes_hip_compress from a host span to a host blob (PCIe both ways):
+    the reference's own call shape, reported beside the device-resident rate."""
+    B.compress_into(perm, FMT)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        blob = B.compress_into(perm, FMT)   # into a fresh host buffer, as a caller gets one
+        times.append(time.perf_counter() - t0)
+    dt = sorted(times)[len(times) // 2]
+    return {"Mentries_per_s": perm.size / dt / 1e6, "ms": dt * 1e3,
+            "device_ms": B.last_device_seconds() * 1e3,
+            "byte_exact": want is not None and hashlib.sha256(blob.tobytes()).hexdigest() == want}
+
+
+def structured(rng, n: int, width: int, near_rate_ppm: int, seed: int) -> np.ndarray:
+    u = rng.random()
+    kind = B.FISHER_YATES if u < 0.6 else (B.SINGLE_CYCLE if u < 0.8 else B.NEAR_IDENTITY)
+    return B.generate(kind, n, seed, near_rate_ppm, width)
+
+
+def batched(count: int, reps: int = 3, seed: int = 4096):
+    """4096 u32 permutations, log-uniform sizes in [2^10, 2^20] (30% shifted
+    by U[1, 256]), 60% Fisher-Yates / 20% single cycle / 20% identity with
+    0.5% transpositions, middle_group_log2 in {4, 5}, separate placement.
+    One batch call against one plain device call per item, inputs resident
+    in HBM for both; every batch blob is compared with the plain one."""
+    rng = np.random.default_rng(seed)
+    perms, fmts = [], []
+    for i in range(count):
+        n = int(2 ** rng.uniform(10, 20))
+        if rng.random() < 0.3:
+            n += int(rng.integers(1, 257))
+        perms.append(structured(rng, n, 4, 5000, seed * 100000 + i))
+        fmts.append(B.FormatOptions(int(rng.integers(4, 6))))
+    ns = [p.size for p in perms]
+    total = sum(ns)
+    offs, pos = [], 0
+    for p in perms:
+        offs.append(pos)
+        pos += (p.nbytes + 7) // 8 * 8
+    host = np.zeros(pos, np.uint8)
+    for p, o in zip(perms, offs):
+        host[o:o + p.nbytes] = p.view(np.uint8)
+    del perms
+    d_in = torch.from_numpy(host).cuda()
+    del host
+    ptrs = [d_in.data_ptr() + o for o in offs]
+    widths = [4] * count
+    out_offs, sizes, need = B.batch_layout(ns, widths, fmts)
+    d_batch = torch.zeros(need, dtype=torch.uint8, device="cuda")
+    d_plain = torch.zeros(need, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def run_batch():
+        return B.compress_batch_device(ptrs, ns, widths, fmts, d_batch.data_ptr(), need, stream)
+
+    def run_plain():
+        for p, n, f, o, sz in zip(ptrs, ns, fmts, out_offs, sizes):
+            B.compress_device(p, n, 4, d_plain.data_ptr() + o, sz, f, stream)
+
+    run_batch()
+    run_plain()
+    torch.cuda.synchronize()
+    tb, tp = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, _, errs, _ = run_batch()
+        torch.cuda.synchronize()
+        tb.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        run_plain()
+        torch.cuda.synchronize()
+        tp.append(time.perf_counter() - t0)
+    tb, tp = min(tb), min(tp)
+    same = bool(torch.equal(d_batch, d_plain)) and all(e is None for e in errs)
+    return {"workload": f"{count} u32 permutations, log-uniform 2^10..2^20 (30% +U[1,256]), "
+                        "60/20/20 Fisher-Yates/single-cycle/near-identity 0.5%, g in {4,5}, separate",
+            "items": count, "entries": total, "seed": seed,
+            "batch_ms": tb * 1e3, "batch_Mentries_per_s": total / tb / 1e6,
+            "plain_ms": tp * 1e3, "plain_Mentries_per_s": total / tp / 1e6,
+            "speedup_vs_plain": tp / tb, "batch_equals_plain_bytes": same}
+
+
+def random_format(rng, n: int) -> "B.FormatOptions":
+    g = int(rng.integers(4, 6))
+    d = B.layout_of(n, B.FormatOptions(g))["routing_depth"]
+
+    def split():
+        if d == 0:
+            return ()
+        parts = int(rng.integers(1, min(3, d) + 1))
+        cuts = sorted(rng.choice(np.arange(1, d), size=parts - 1, replace=False)) if parts > 1 else []
+        bounds = [0, *cuts, d]
+        return tuple(int(bounds[i + 1] - bounds[i]) for i in range(parts))
+
+    return B.FormatOptions(g, split(), split(), int(rng.integers(0, 3)))
+
+
+def served(count: int, rate_rps: float, seed: int = 5005, lanes: int = 3, max_batch: int = 64):
+    """Requests of log-uniform sizes in [2^8, 2^24], 80% u32 / 20% u64, the
+    batched structure mix, random valid cluster splits and placements; host
+    arrays in, host blobs out.  Poisson arrivals at `rate_rps` into the
+    request loop (64 GB budget) give p50/p99 latency from submit to blob; a
+    burst of all requests gives its peak rate; one plain benes_hip_compress
+    per request, one after another, is the comparison."""
+    import threading
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        n = int(2 ** rng.uniform(8, 24))
+        width = 8 if rng.random() < 0.2 else 4
+        reqs.append((structured(rng, n, width, 5000, seed * 100000 + i), random_format(rng, n)))
+    total = sum(p.size for p, _ in reqs)
+    gaps = rng.exponential(1.0 / rate_rps, size=count)
+
+    # Plain: one call per request, sequentially.
+    blobs_plain = []
+    lat_plain = []
+    B.compress_into(*reqs[0])
+    t0 = time.perf_counter()
+    for p, f in reqs:
+        t1 = time.perf_counter()
+        blobs_plain.append(B.compress_into(p, f))
+        lat_plain.append(time.perf_counter() - t1)
+    t_plain = time.perf_counter() - t0
+
+    def run(arrivals):
+        with B.Server(budget_bytes=64 << 30, max_batch=max_batch, lanes=lanes) as srv:
+            ids = [None] * count
+
+            def producer():
+                start = time.perf_counter()
+                due = 0.0
+                for i, (p, f) in enumerate(reqs):
+                    if arrivals is not None:
+                        due += arrivals[i]
+                        wait = start + due - time.perf_counter()
+                        if wait > 0:
+                            time.sleep(wait)
+                    ids[i] = srv.submit(p, f)
+
+            t0 = time.perf_counter()
+            th = threading.Thread(target=producer)
+            th.start()
+            th.join()
+            results = [srv.wait(i) for i in ids]
+            wall = time.perf_counter() - t0
+            stats = srv.stats()
+        lats = sorted(lat for _, lat in results)
+        same = all(not isinstance(b, Exception) and np.array_equal(b, bp)
+                   for (b, _), bp in zip(results, blobs_plain))
+        return wall, lats, same, stats
+
+    wall_p, lats_p, same_p, st_p = run(gaps)
+    wall_b, lats_b, same_b, st_b = run(None)
+    lp = sorted(lat_plain)
+    q = lambda xs, f: xs[min(len(xs) - 1, int(f * len(xs)))] * 1e3
+    return {"workload": f"{count} requests, log-uniform 2^8..2^24, 80% u32 / 20% u64, "
+                        "60/20/20 structures, random cluster splits and placements",
+            "requests": count, "entries": total, "seed": seed, "lanes": lanes,
+            "max_batch": max_batch, "hbm_budget_bytes": 64 << 30,
+            "poisson": {"offered_rps": rate_rps, "wall_s": wall_p,
+                        "Mentries_per_s": total / wall_p / 1e6, "p50_ms": q(lats_p, 0.5),
+                        "p99_ms": q(lats_p, 0.99), "batches": st_p["batches"],
+                        "max_concurrent_batches": st_p["max_concurrent_batches"],
+                        "peak_bytes": st_p["peak_bytes"]},
+            "burst": {"wall_s": wall_b, "Mentries_per_s": total / wall_b / 1e6,
+                      "p50_ms": q(lats_b, 0.5), "p99_ms": q(lats_b, 0.99),
+                      "batches": st_b["batches"]},
+            "plain": {"wall_s": t_plain, "Mentries_per_s": total / t_plain / 1e6,
+                      "p50_ms": q(lp, 0.5), "p99_ms": q(lp, 0.99)},
+            "burst_speedup_vs_plain": t_plain / wall_b,
+            "each_result_equals_plain": same_p and same_b}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the 2^26 Fisher-Yates line (no other inputs, batch or serving runs)")
+    ap.add_argument("--batch-items", type=int, default=4096)
+    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--offered-rps", type=float, default=400.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -156,7 +370,7 @@ def main():
     prof_steps = 3
     for _ in range(prof_steps):
         step()
-    stats = B.kernel_stats()
+    stats = {k: v for k, v in B.kernel_stats().items() if k.startswith("k_")}
     B.set_profiling(False)
     total_ms = sum(ms for ms, _ in stats.values())
     dom = max(stats, key=lambda k: stats[k][0])
@@ -210,6 +424,11 @@ def main():
         },
         "setup": {"generate_s": gen_s},
     }
+    if not args.no_extra:
+        line["inputs_2e26"] = other_inputs(d_out, size, stream, max(3, args.steps // 2))
+        line["host_inclusive"] = host_inclusive(perm, want)
+        line["batched"] = batched(args.batch_items)
+        line["served"] = served(args.requests, args.offered_rps)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(perm, want)
     print(json.dumps(line), flush=True)

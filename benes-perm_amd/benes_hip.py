@@ -208,28 +208,33 @@ def encoded_size(n: int, fmt: FormatOptions | None = None) -> int:
     return out.value
 
 
-def compress(perm: np.ndarray, fmt: FormatOptions | None = None) -> bytes:
-    """Compresses a host uint32/uint64 permutation on the GPU."""
+def compress_into(perm: np.ndarray, fmt: FormatOptions | None = None,
+                  out: np.ndarray | None = None) -> np.ndarray:
+    """Compresses a host uint32/uint64 permutation on the GPU into `out`
+    (a uint8 array of at least encoded_size bytes, allocated when None);
+    returns the blob as a view of it."""
     perm = np.ascontiguousarray(perm)
     if perm.dtype not in (np.uint32, np.uint64):
         raise TypeError("permutation must be uint32 or uint64")
     f, keep = _format(fmt)
+    if out is None:
+        out = np.empty(encoded_size(perm.size, fmt), np.uint8)
     lib = library()
     err = ctypes.create_string_buffer(256)
     size = ctypes.c_uint64()
-    rc = lib.benes_hip_compress(perm.ctypes.data_as(ctypes.c_void_p), perm.size,
-                                perm.dtype.itemsize, ctypes.byref(f), None, 0,
-                                ctypes.byref(size), None, err, 256)
-    if rc != BUFFER_TOO_SMALL:
-        _raise(rc, err)
-    out = np.empty(size.value, np.uint8)
     secs = ctypes.c_double()
     rc = lib.benes_hip_compress(perm.ctypes.data_as(ctypes.c_void_p), perm.size,
                                 perm.dtype.itemsize, ctypes.byref(f),
                                 out.ctypes.data_as(ctypes.c_void_p), out.size,
                                 ctypes.byref(size), ctypes.byref(secs), err, 256)
     _raise(rc, err)
-    return out.tobytes()
+    return out[:size.value]
+
+
+def compress(perm: np.ndarray, fmt: FormatOptions | None = None) -> bytes:
+    """Compresses a host uint32/uint64 permutation on the GPU (the blob as
+    bytes, as the reference's compress returns it by value)."""
+    return compress_into(perm, fmt).tobytes()
 
 
 def compress_device(d_perm_ptr: int, n: int, width: int, d_out_ptr: int, capacity: int,
@@ -328,7 +333,7 @@ class Server:
         perm = np.ascontiguousarray(perm)
         f, keep = _format(fmt)
         size = encoded_size(perm.size, fmt)
-        out = np.zeros(size, np.uint8)
+        out = np.empty(size, np.uint8)
         status = ctypes.c_int()
         err = ctypes.create_string_buffer(256)
         rid = library().benes_hip_server_submit(self._h, perm.ctypes.data, perm.size,
@@ -341,8 +346,9 @@ class Server:
         return rid
 
     def wait(self, rid: int, timeout_ms: int = -1):
-        """(blob bytes or the exception of the request, latency seconds);
-        raises TimeoutError, keeping the request, when timeout_ms runs out."""
+        """(blob as a uint8 array, or the exception of the request; latency
+        seconds); raises TimeoutError, keeping the request, when timeout_ms
+        runs out."""
         size = ctypes.c_uint64()
         lat = ctypes.c_double()
         err = ctypes.create_string_buffer(256)
@@ -353,7 +359,7 @@ class Server:
         perm, out, keep = self._pending.pop(rid)
         if rc != OK:
             return _ERRORS.get(rc, RuntimeError)(err.value.decode()), lat.value
-        return out[:size.value].tobytes(), lat.value
+        return out[:size.value], lat.value
 
     def stats(self) -> dict:
         c = _Counters()

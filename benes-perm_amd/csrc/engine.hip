@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/benes_hip.h"
+#include "host_runtime.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 #include "status.hpp"
@@ -52,21 +53,16 @@ struct Pending {
     hipEvent_t begin, end;
 };
 
-struct HipError {
-    hipError_t code;
-    const char* what;
-};
-
-#define BH_CHECK(expr)                                       \
-    do {                                                     \
-        const hipError_t e_ = (expr);                        \
-        if (e_ != hipSuccess) throw HipError{e_, #expr};     \
-    } while (0)
-
 // Grow-only device buffer.
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (ptr) (void)hipFree(ptr);
+    }
     void ensure(size_t need) {
         if (need <= bytes) return;
         if (ptr) BH_CHECK(hipFree(ptr));
@@ -90,6 +86,18 @@ struct Workspace {
     DevBuf input, level[2], ilevel[2], inv0, uf, srows, trows, desc, plans, items, segs, errs,
         blob;
     std::vector<int> host_errs;
+    PinnedRing ring;  // host <-> device staging of this workspace's stream
+    Workspace() = default;
+    Workspace(const Workspace&) = delete;
+    Workspace& operator=(const Workspace&) = delete;
+    ~Workspace() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        if (ev_begin) (void)hipEventDestroy(ev_begin);
+        if (ev_end) (void)hipEventDestroy(ev_end);
+    }
     void init(int dev) {
         if (stream) return;
         device = dev;
@@ -155,6 +163,23 @@ struct Profiler {
         pending.clear();
     }
 };
+
+// Host-side stages (staging copies) join the same table while profiling,
+// named "host:<stage>", with their wall time.
+void note_host_time(const char* name, std::chrono::steady_clock::time_point since) {
+    const double ms = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - since).count();
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    if (!g_profiling) return;
+    auto it = std::find_if(g_prof_stats.begin(), g_prof_stats.end(),
+                           [&](const KernelStat& k) { return k.name == name; });
+    if (it == g_prof_stats.end()) {
+        g_prof_stats.push_back(KernelStat{name, 0, 0});
+        it = g_prof_stats.end() - 1;
+    }
+    it->ms += ms;
+    it->launches += 1;
+}
 
 #define BH_LAUNCH(prof, name, ...)       \
     do {                                 \
@@ -561,16 +586,21 @@ int compress_one(const void* perm, bool on_device, uint64_t n, uint32_t width,
     }
     with_workspace(st, [&](Workspace& ws) {
         if (!on_device) {
+            const auto t0 = std::chrono::steady_clock::now();
             ws.input.ensure(n * width);
-            BH_CHECK(hipMemcpyAsync(ws.input.ptr, perm, n * width, hipMemcpyHostToDevice,
-                                    ws.stream));
+            ws.ring.upload(ws.input.as<uint8_t>(),
+                           {HostPiece{static_cast<uint8_t*>(const_cast<void*>(perm)), 0, n * width}},
+                           ws.stream);
             jobs[0].dsrc = ws.input.ptr;
+            note_host_time("host:upload", t0);
         }
         // Write straight into the caller's buffer when it can take whole words.
         const bool direct = dst_dev && (reinterpret_cast<uintptr_t>(dst_dev) & 7) == 0 &&
                             capacity >= round8(end);
         std::vector<int> flags;
+        const auto t_run = std::chrono::steady_clock::now();
         run_batch(ws, jobs, direct ? dst_dev : nullptr, user_stream, flags, device_seconds);
+        note_host_time("host:run", t_run);
         if (flags[0]) {
             st.set(kInvalidArgument, kNotPermutation);
             return;
@@ -581,7 +611,9 @@ int compress_one(const void* perm, bool on_device, uint64_t n, uint32_t width,
             BH_CHECK(hipStreamSynchronize(ws.stream));
         }
         if (dst_host) {
-            BH_CHECK(hipMemcpy(dst_host, ws.blob.ptr, end, hipMemcpyDeviceToHost));
+            const auto t0 = std::chrono::steady_clock::now();
+            ws.ring.download(ws.blob.as<uint8_t>(), {HostPiece{dst_host, 0, end}}, ws.stream);
+            note_host_time("host:download", t0);
         }
     });
     return finish(st, err, err_len);
@@ -620,14 +652,15 @@ void mark_invalid(benes_hip_item& item) {
 // ---------------------------------------------------------------------------
 // Request loop (continuous batching)
 //
-// Requests queue in arrival order.  A scheduler thread owns `lanes` streams,
-// each with its own workspace (the routing state of its batch stays in HBM).
-// Whenever a lane is idle it takes the longest prefix of the queue that fits
-// the lane's share of the HBM budget and the batch cap, copies the inputs to
-// pinned staging, and queues the whole batch on its stream without waiting;
-// requests that arrive meanwhile are picked up by the next free lane, so
-// batches on different lanes overlap on the device.  A finished batch copies
-// each blob to its request's buffer and wakes its waiters.
+// Requests queue in arrival order.  The server owns `lanes` lanes, each a
+// stream with its own workspace (the routing state of its batch stays in
+// HBM), its own pinned staging ring and its own host thread.  A scheduler
+// thread hands the longest prefix of the queue that fits a lane's share of
+// the HBM budget and the batch cap to the first idle lane; the lane thread
+// stages the inputs, queues the whole batch, drains the blobs into the
+// requests' buffers and wakes their waiters.  Requests that arrive meanwhile
+// go to the next idle lane, so the staging of one batch overlaps the device
+// work of another, and batches on different lanes overlap on the device.
 
 struct Request {
     uint64_t id = 0;
@@ -643,30 +676,16 @@ struct Request {
 
 struct Lane {
     Workspace ws;
-    std::vector<Request*> batch;
+    std::vector<Request*> batch;  // set by the scheduler, emptied by the lane thread
     std::vector<Job> jobs;
-    uint8_t* pinned_in = nullptr;
-    size_t pinned_in_bytes = 0;
-    uint8_t* pinned_out = nullptr;
-    size_t pinned_out_bytes = 0;
-    int* pinned_flags = nullptr;
-    size_t pinned_flags_count = 0;
-    hipEvent_t done = nullptr;
-    bool busy = false;
-    uint64_t held = 0;  // budget bytes of the batch in flight
+    std::thread thread;
+    std::condition_variable cv;  // wakes the lane thread (waits on Server::mu)
+    bool busy = false;           // a batch is assigned and not yet finished
+    uint64_t held = 0;           // budget bytes of the batch in flight
 };
 
 uint64_t request_footprint(uint64_t n, uint32_t width, uint64_t blob_bytes) {
     return n * (width + 28) + n / 4 + round8(blob_bytes) + (64u << 10);
-}
-
-void ensure_pinned(uint8_t*& p, size_t& have, size_t need) {
-    if (need <= have) return;
-    if (p) BH_CHECK(hipHostFree(p));
-    p = nullptr;
-    have = 0;
-    BH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), need, hipHostMallocDefault));
-    have = need;
 }
 
 }  // namespace
@@ -678,7 +697,8 @@ struct Server {
     std::unordered_map<uint64_t, Request*> requests;
     std::vector<Lane*> lanes;
     std::thread thread;
-    bool stop = false;
+    bool stop = false;        // no more requests: drain and end the scheduler
+    bool stop_lanes = false;  // end the lane threads (after the scheduler)
     int device = 0;
     uint64_t budget = 0, lane_cap = 0, max_batch = 0, next_id = 1;
     uint64_t in_flight_bytes = 0, peak_bytes = 0, batches = 0, completed = 0, entries = 0;
@@ -686,17 +706,14 @@ struct Server {
     std::string fatal;
 
     void run();
-    bool launch(Lane& lane);
-    void complete(Lane& lane);
+    bool assign(Lane& lane);
+    void serve(Lane& lane);
+    void process(Lane& lane);
 };
 
-namespace {
-
-}  // namespace
-
-bool Server::launch(Lane& lane) {
+bool Server::assign(Lane& lane) {
     // Caller holds mu.  Take the queue prefix that fits this lane.
-    uint64_t bytes = 0, in_bytes = 0, out_bytes = 0;
+    uint64_t bytes = 0;
     lane.batch.clear();
     while (!queue.empty() && lane.batch.size() < max_batch) {
         Request* r = queue.front();
@@ -704,8 +721,6 @@ bool Server::launch(Lane& lane) {
             break;
         }
         bytes += r->footprint;
-        in_bytes += round8(r->job.n * r->job.width);
-        out_bytes += round8(r->job.lay.plan.end_offset);
         lane.batch.push_back(r);
         queue.pop_front();
     }
@@ -715,120 +730,117 @@ bool Server::launch(Lane& lane) {
     peak_bytes = std::max(peak_bytes, in_flight_bytes);
     ++batches;
     lane.busy = true;
-    std::vector<Request*> batch = lane.batch;
-    mu.unlock();
-    try {
-        BH_CHECK(hipSetDevice(device));
-        lane.ws.init(device);
-        ensure_pinned(lane.pinned_in, lane.pinned_in_bytes, in_bytes);
-        ensure_pinned(lane.pinned_out, lane.pinned_out_bytes, out_bytes);
-        if (lane.pinned_flags_count < batch.size()) {
-            if (lane.pinned_flags) BH_CHECK(hipHostFree(lane.pinned_flags));
-            BH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&lane.pinned_flags),
-                                   sizeof(int) * batch.size(), hipHostMallocDefault));
-            lane.pinned_flags_count = batch.size();
-        }
-        lane.ws.input.ensure(in_bytes);
-        lane.jobs.clear();
-        uint64_t at = 0;
-        for (Request* r : batch) {
-            const uint64_t len = r->job.n * r->job.width;
-            std::memcpy(lane.pinned_in + at, r->host_perm, len);
-            Job j = r->job;
-            j.dsrc = lane.ws.input.as<uint8_t>() + at;
-            lane.jobs.push_back(std::move(j));
-            at += round8(len);
-        }
-        const hipStream_t s = lane.ws.stream;
-        BH_CHECK(hipMemcpyAsync(lane.ws.input.ptr, lane.pinned_in, at, hipMemcpyHostToDevice, s));
-        Profiler prof;
-        prof.stream = s;
-        enqueue_batch(lane.ws, lane.jobs, prof, nullptr);
-        BH_CHECK(hipMemcpyAsync(lane.pinned_out, lane.ws.blob.ptr, out_bytes,
-                                hipMemcpyDeviceToHost, s));
-        BH_CHECK(hipMemcpyAsync(lane.pinned_flags, lane.ws.errs.ptr, sizeof(int) * batch.size(),
-                                hipMemcpyDeviceToHost, s));
-        if (!lane.done) BH_CHECK(hipEventCreateWithFlags(&lane.done, hipEventDisableTiming));
-        BH_CHECK(hipEventRecord(lane.done, s));
-    } catch (const HipError& e) {
-        mu.lock();
-        fatal = std::string("hip error: ") + hipGetErrorString(e.code) + " in " + e.what;
-        return true;
-    } catch (const std::bad_alloc&) {
-        mu.lock();
-        fatal = "allocation failed";
-        return true;
-    }
-    mu.lock();
+    lane.cv.notify_one();
     return true;
 }
 
-void Server::complete(Lane& lane) {
-    // Caller holds mu; the lane's event has completed.
-    const auto now = std::chrono::steady_clock::now();
-    for (size_t k = 0; k < lane.batch.size(); ++k) {
-        Request* r = lane.batch[k];
-        if (lane.pinned_flags[k]) {
+// Runs one batch on the lane (without holding mu): stage in, queue, drain.
+void Server::process(Lane& lane) {
+    std::vector<Request*>& batch = lane.batch;
+    BH_CHECK(hipSetDevice(device));
+    lane.ws.init(device);
+    uint64_t in_bytes = 0;
+    for (Request* r : batch) in_bytes += round8(r->job.n * r->job.width);
+    lane.ws.input.ensure(in_bytes);
+    lane.ws.errs.ensure(sizeof(int) * batch.size());
+    lane.jobs.clear();
+    std::vector<HostPiece> in;
+    uint64_t at = 0;
+    for (Request* r : batch) {
+        const uint64_t len = r->job.n * r->job.width;
+        in.push_back(HostPiece{static_cast<uint8_t*>(const_cast<void*>(r->host_perm)), at, len});
+        Job j = r->job;
+        j.dsrc = lane.ws.input.as<uint8_t>() + at;
+        lane.jobs.push_back(std::move(j));
+        at += round8(len);
+    }
+    const hipStream_t s = lane.ws.stream;
+    lane.ws.ring.upload(lane.ws.input.as<uint8_t>(), in, s);
+    Profiler prof;
+    prof.stream = s;
+    enqueue_batch(lane.ws, lane.jobs, prof, nullptr);
+    std::vector<int> flags(batch.size());
+    BH_CHECK(hipMemcpyAsync(flags.data(), lane.ws.errs.ptr, sizeof(int) * batch.size(),
+                            hipMemcpyDeviceToHost, s));
+    BH_CHECK(hipStreamSynchronize(s));
+    std::vector<HostPiece> out;
+    for (size_t k = 0; k < batch.size(); ++k) {
+        Request* r = batch[k];
+        const uint64_t end = r->job.lay.plan.end_offset;
+        if (flags[k]) {
             r->status = kInvalidArgument;
             r->message = kNotPermutation;
-        } else if (!r->out || r->out_capacity < r->job.lay.plan.end_offset) {
+        } else if (!r->out || r->out_capacity < end) {
             r->status = kBufferTooSmall;
             r->message = "output buffer is too small";
         } else {
-            std::memcpy(r->out, lane.pinned_out + lane.jobs[k].blob * 8,
-                        r->job.lay.plan.end_offset);
+            out.push_back(HostPiece{r->out, lane.jobs[k].blob * 8, end});
             r->status = kOk;
-            entries += r->job.n;
         }
-        r->done = now;
-        ++completed;
     }
-    lane.busy = false;
-    in_flight_bytes -= lane.held;
-    lane.held = 0;
-    lane.batch.clear();
-    cv_done.notify_all();
+    lane.ws.ring.download(lane.ws.blob.as<uint8_t>(), out, s);
+}
+
+void Server::serve(Lane& lane) {
+    std::unique_lock<std::mutex> lock(mu);
+    while (true) {
+        lane.cv.wait(lock, [&] { return stop_lanes || !lane.batch.empty(); });
+        if (lane.batch.empty()) break;  // stop_lanes with nothing assigned
+        std::string error;
+        lock.unlock();
+        try {
+            process(lane);
+        } catch (const HipError& e) {
+            error = std::string("hip error: ") + hipGetErrorString(e.code) + " in " + e.what;
+        } catch (const std::bad_alloc&) {
+            error = "allocation failed";
+        }
+        lock.lock();
+        const auto now = std::chrono::steady_clock::now();
+        for (Request* r : lane.batch) {
+            if (!error.empty()) {
+                r->status = kRuntime;
+                r->message = error;
+            } else if (r->status == kOk) {
+                entries += r->job.n;
+            }
+            r->done = now;
+            ++completed;
+        }
+        if (!error.empty() && fatal.empty()) fatal = error;
+        lane.batch.clear();
+        lane.busy = false;
+        in_flight_bytes -= lane.held;
+        lane.held = 0;
+        cv_done.notify_all();
+        cv_work.notify_one();
+    }
 }
 
 void Server::run() {
     std::unique_lock<std::mutex> lock(mu);
     while (true) {
-        if (stop && queue.empty() &&
-            std::none_of(lanes.begin(), lanes.end(), [](Lane* l) { return l->busy; })) {
-            break;
-        }
-        bool progressed = false;
-        for (Lane* lane : lanes) {
-            if (lane->busy && hipEventQuery(lane->done) == hipSuccess) {
-                complete(*lane);
-                progressed = true;
-            }
-        }
-        for (Lane* lane : lanes) {
-            if (!lane->busy && !queue.empty() && fatal.empty()) {
-                if (launch(*lane)) progressed = true;
-            }
-        }
-        uint64_t busy = 0;
-        for (Lane* lane : lanes) busy += lane->busy ? 1 : 0;
-        max_concurrent = std::max(max_concurrent, busy);
+        const bool any_busy =
+            std::any_of(lanes.begin(), lanes.end(), [](Lane* l) { return l->busy; });
+        if (stop && queue.empty() && !any_busy) break;
         if (!fatal.empty()) {
-            // Fail everything still waiting; in-flight lanes finish normally.
+            // Fail everything still waiting; lanes in flight finish normally.
             for (Request* r : queue) {
                 r->status = kRuntime;
                 r->message = fatal;
                 r->done = std::chrono::steady_clock::now();
+                ++completed;
             }
             queue.clear();
             cv_done.notify_all();
         }
-        if (!progressed) {
-            if (busy) {
-                cv_work.wait_for(lock, std::chrono::microseconds(50));
-            } else {
-                cv_work.wait(lock, [&] { return stop || !queue.empty(); });
-            }
+        for (Lane* lane : lanes) {
+            if (!lane->busy && !queue.empty()) assign(*lane);
         }
+        uint64_t busy = 0;
+        for (Lane* lane : lanes) busy += lane->busy ? 1 : 0;
+        max_concurrent = std::max(max_concurrent, busy);
+        cv_work.wait(lock);
     }
 }
 
@@ -900,13 +912,16 @@ int benes_hip_compress_batch(benes_hip_item* items, uint64_t count, double* devi
             bytes += round8(jobs[k].n * jobs[k].width);
         }
         ws.input.ensure(bytes);
+        std::vector<HostPiece> in;
         for (size_t k = 0; k < jobs.size(); ++k) {
-            BH_CHECK(hipMemcpyAsync(ws.input.as<uint8_t>() + at[k], jobs[k].dsrc,
-                                    jobs[k].n * jobs[k].width, hipMemcpyHostToDevice, ws.stream));
+            in.push_back(HostPiece{static_cast<uint8_t*>(const_cast<void*>(jobs[k].dsrc)), at[k],
+                                   jobs[k].n * jobs[k].width});
             jobs[k].dsrc = ws.input.as<uint8_t>() + at[k];
         }
+        ws.ring.upload(ws.input.as<uint8_t>(), in, ws.stream);
         std::vector<int> flags;
         run_batch(ws, jobs, nullptr, nullptr, flags, device_seconds);
+        std::vector<HostPiece> out;
         for (size_t k = 0; k < jobs.size(); ++k) {
             benes_hip_item& item = items[index[k]];
             if (flags[k]) {
@@ -917,10 +932,9 @@ int benes_hip_compress_batch(benes_hip_item* items, uint64_t count, double* devi
                 item.status = BENES_HIP_BUFFER_TOO_SMALL;
                 continue;
             }
-            BH_CHECK(hipMemcpyAsync(item.out, ws.blob.as<uint8_t>() + jobs[k].blob * 8,
-                                    item.out_size, hipMemcpyDeviceToHost, ws.stream));
+            out.push_back(HostPiece{item.out, jobs[k].blob * 8, item.out_size});
         }
-        BH_CHECK(hipStreamSynchronize(ws.stream));
+        ws.ring.download(ws.blob.as<uint8_t>(), out, ws.stream);
     });
     return finish(st, err, err_len);
 }
@@ -1069,6 +1083,7 @@ benes_hip_server* benes_hip_server_create(uint64_t hbm_budget_bytes, uint32_t ma
     srv->lane_cap = hbm_budget_bytes / lanes;
     srv->max_batch = std::max<uint32_t>(1, max_batch);
     for (uint32_t i = 0; i < lanes; ++i) srv->lanes.push_back(new Lane());
+    for (Lane* lane : srv->lanes) lane->thread = std::thread([srv, lane] { srv->serve(*lane); });
     srv->thread = std::thread([srv] { srv->run(); });
     return srv;
 }
@@ -1151,15 +1166,17 @@ void benes_hip_server_destroy(benes_hip_server* srv) {
         srv->stop = true;
         srv->cv_work.notify_all();
     }
-    srv->thread.join();
-    for (auto& kv : srv->requests) delete kv.second;
+    srv->thread.join();  // returns once the queue is empty and no lane is busy
+    {
+        std::lock_guard<std::mutex> lock(srv->mu);
+        srv->stop_lanes = true;
+        for (Lane* lane : srv->lanes) lane->cv.notify_all();
+    }
     for (Lane* lane : srv->lanes) {
-        if (lane->pinned_in) (void)hipHostFree(lane->pinned_in);
-        if (lane->pinned_out) (void)hipHostFree(lane->pinned_out);
-        if (lane->pinned_flags) (void)hipHostFree(lane->pinned_flags);
-        if (lane->done) (void)hipEventDestroy(lane->done);
+        lane->thread.join();
         delete lane;
     }
+    for (auto& kv : srv->requests) delete kv.second;
     delete srv;
 }
 

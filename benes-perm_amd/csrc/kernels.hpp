@@ -607,10 +607,48 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
         const uint32_t nsets = one_set ? 1u : (1u << (R - d0));
         const uint32_t set_nodes = one_set ? (1u << k) : (1u << r);
         const uint32_t packets = static_cast<uint32_t>((maxpairs + span - 1) >> span_log2);
+        const uint32_t jobs = nsets * packets;
+        if ((static_cast<uint64_t>(set_nodes) << span_log2) <= 64) {
+            // Short runs: one lane per (set, packet).  Within a cluster group
+            // the nodes' shares of one packet follow each other in path
+            // order, so the run starts at the first internal node's address
+            // and each later node continues where the previous one ended.
+            for (uint32_t job = threadIdx.x; job < jobs; job += blockDim.x) {
+                const uint32_t set = job / packets, pk = job - set * packets;
+                const uint32_t first = set * set_nodes, end = first + set_nodes;
+                uint32_t f = first;
+                while (f < end && sub_count(n0, k, f) <= L) ++f;
+                if (f == end) continue;
+                const uint64_t path = (b0 << k) | f;
+                const SideAddr a =
+                    address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
+                const uint32_t first_pair = pk << span_log2;
+                uint64_t acc = 0;
+                uint32_t len = 0;
+                for (uint32_t pi = f; pi < end; ++pi) {
+                    const uint32_t share = static_cast<uint32_t>(
+                        packet_share(sub_count(n0, k, pi), pk, span_log2, L));
+                    const uint32_t pos0 = noff[pi] + first_pair;
+                    for (uint32_t b = 0; b < share; ++b) {
+                        const uint32_t pos = pos0 + b;
+                        acc |= ((rows[pos >> 6] >> (pos & 63)) & 1) << len;
+                        ++len;
+                    }
+                }
+                if (acc == 0) continue;
+                const uint64_t dst = secbase + bit_of(a, first_pair);
+                auto* bw = reinterpret_cast<unsigned long long*>(blob) + (dst >> 6);
+                const uint32_t sh = dst & 63;
+                atomicOr(bw, static_cast<unsigned long long>(acc << sh));
+                if (sh && (acc >> (64 - sh))) {
+                    atomicOr(bw + 1, static_cast<unsigned long long>(acc >> (64 - sh)));
+                }
+            }
+            continue;
+        }
         SideAddr cached{};
         uint32_t cached_set = ~0u;
         // Each wave takes a contiguous range of jobs, so it revisits one set.
-        const uint32_t jobs = nsets * packets;
         const uint32_t per_wave = (jobs + nwaves - 1) / nwaves;
         const uint32_t job_end = min(jobs, (wave + 1) * per_wave);
         for (uint32_t job = wave * per_wave; job < job_end; ++job) {

@@ -63,7 +63,7 @@ def test_results_same_alone_and_in_the_mix():
         blob, latency = results[i]
         assert not isinstance(blob, Exception), blob
         assert latency > 0
-        assert blob == B.compress(perm, fmt), i
+        assert blob.tobytes() == B.compress(perm, fmt), i
     assert stats["completed"] == len(reqs)
     assert stats["batches"] < len(reqs)            # requests were batched
     assert stats["peak_bytes"] <= stats["budget_bytes"]
@@ -73,7 +73,7 @@ def test_small_results_match_oracle():
     reqs = request_set(2, 40, 13)
     with B.Server(budget_bytes=1 << 30, max_batch=16, lanes=3) as srv:
         ids = [srv.submit(p, f) for p, f in reqs]
-        out = [srv.wait(i)[0] for i in ids]
+        out = [srv.wait(i)[0].tobytes() for i in ids]
     for (perm, fmt), blob in zip(reqs, out):
         assert blob == S.oracle_compress(perm, fmt.middle_group_log2, fmt.middle_placement,
                                          fmt.input_clusters, fmt.output_clusters)
@@ -90,6 +90,6 @@ def test_budget_and_bad_requests():
         blob, _ = srv.wait(bad)
         assert isinstance(blob, ValueError)
         blob, _ = srv.wait(good)
-        assert blob == S.oracle_compress(B.generate(B.FISHER_YATES, 5000, 2))
+        assert blob.tobytes() == S.oracle_compress(B.generate(B.FISHER_YATES, 5000, 2))
         with pytest.raises(ValueError):
             srv.submit(np.zeros(0, np.uint32))
