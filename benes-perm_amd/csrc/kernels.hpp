@@ -498,8 +498,9 @@ __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ ite
 //
 // One workgroup owns a subtree of at most kSub entries (a node at the fusion
 // depth d0) and runs every remaining level in LDS: node-local permutation
-// (u16, ping-pong), union-find words (u32), and the level's switch values as
-// bit arrays indexed like the permutation.  Work is dealt as (node, 64-entry
+// (u16, ping-pong; while a level is coloured, the other half holds the
+// nodes' inverses), union-find words over input slots (u32), and the level's
+// switch values as bit arrays indexed like the permutation.  Work is dealt as (node, 64-entry
 // chunk) items to waves, so each wave stays inside one node and the node's
 // offset comes from a per-level LDS table.
 
@@ -769,7 +770,8 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const ItemCtx* __restri
             tbits[i] = 0;
         }
         __syncthreads();
-        // Union: items are (node, 64-entry chunk).
+        // The node's inverse, held in the other half of the ping-pong buffer
+        // until routing overwrites it.  Items are (node, 64-entry chunk).
         const uint32_t cpn = (cmax + 63) >> 6;
         for (uint32_t item = wave; item < nodes * cpn; item += nwaves) {
             const uint32_t pi = item / cpn, chunk = item - pi * cpn;
@@ -778,46 +780,82 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const ItemCtx* __restri
             const uint32_t x = chunk * 64 + lane;
             if (x >= c) continue;
             const uint32_t off = noff[pi];
-            const uint32_t v = perm[off + x];
-            if ((c & 1) && v == c - 1) continue;
-            uf_union_edge<GroupForest>(uf + off, input_slot<uint32_t>(x, c),
-                                       output_pair_slot<uint32_t>(v >> 1, c), (x ^ v) & 1u);
+            next[off + perm[off + x]] = static_cast<uint16_t>(x);
         }
         __syncthreads();
-        // Resolve and route: items are (node, 64-pair chunk); the odd tail
-        // entry rides with pair index h.
+        // Union of the two input slots reaching each output pair (the same
+        // input-slot forest as the HBM levels).  Items are (node, 64-pair
+        // chunk).
         const uint32_t ppn = ((cmax >> 1) + 1 + 63) >> 6;
+        for (uint32_t item = wave; item < nodes * ppn; item += nwaves) {
+            const uint32_t pi = item / ppn, chunk = item - pi * ppn;
+            const uint32_t c = sub_count(n0, k, pi);
+            if (c <= L) continue;
+            const uint32_t q = chunk * 64 + lane;
+            if (q >= (c >> 1)) continue;
+            const uint32_t off = noff[pi];
+            const uint32_t a = next[off + 2 * q], b = next[off + 2 * q + 1];
+            const uint32_t sa = input_slot<uint32_t>(a, c), sb = input_slot<uint32_t>(b, c);
+            if (sa != sb) uf_union_slots<GroupForest>(uf + off, sa, sb, 1u ^ ((a ^ b) & 1u));
+        }
+        __syncthreads();
+        // Input switch values: each pair's parity to its root.
         for (uint32_t item = wave; item < nodes * ppn; item += nwaves) {
             const uint32_t pi = item / ppn, chunk = item - pi * ppn;
             const uint32_t c = sub_count(n0, k, pi);
             if (c <= L) continue;  // wave-uniform
             const uint32_t off = noff[pi], h = c >> 1;
             const uint32_t p = chunk * 64 + lane;
-            uint32_t s = 0, t = 0;
+            uint32_t s = 0;
+            if (p < h) uf_find<GroupForest>(uf + off, input_pair_slot<uint32_t>(p, c), s);
+            const uint64_t ms = __ballot(s != 0);
+            if (lane == 0 && ms) {
+                const uint32_t pos = off + chunk * 64, w = pos >> 6, sh = pos & 63;
+                atomicOr(reinterpret_cast<unsigned long long*>(sbits + w), ms << sh);
+                if (sh) atomicOr(reinterpret_cast<unsigned long long*>(sbits + w + 1),
+                                 ms >> (64 - sh));
+            }
+        }
+        __syncthreads();
+        // Output switch values t_q = colour(I[2q]), colour(x) = s_(x>>1) ^
+        // (x & 1) and 0 for the odd tail; then the children of P.
+        for (uint32_t item = wave; item < nodes * ppn; item += nwaves) {
+            const uint32_t pi = item / ppn, chunk = item - pi * ppn;
+            const uint32_t c = sub_count(n0, k, pi);
+            if (c <= L) continue;  // wave-uniform
+            const uint32_t off = noff[pi], h = c >> 1;
+            const uint32_t q = chunk * 64 + lane;
+            uint32_t t = 0;
+            if (q < h) {
+                const uint32_t a0 = next[off + 2 * q];
+                if (!((c & 1) && a0 == c - 1)) {
+                    const uint32_t pos = off + (a0 >> 1);
+                    t = static_cast<uint32_t>((sbits[pos >> 6] >> (pos & 63)) & 1) ^ (a0 & 1u);
+                }
+            }
+            const uint64_t mt = __ballot(t != 0);
+            if (lane == 0 && mt) {
+                const uint32_t pos = off + chunk * 64, w = pos >> 6, sh = pos & 63;
+                atomicOr(reinterpret_cast<unsigned long long*>(tbits + w), mt << sh);
+                if (sh) atomicOr(reinterpret_cast<unsigned long long*>(tbits + w + 1),
+                                 mt >> (64 - sh));
+            }
+        }
+        __syncthreads();
+        for (uint32_t item = wave; item < nodes * ppn; item += nwaves) {
+            const uint32_t pi = item / ppn, chunk = item - pi * ppn;
+            const uint32_t c = sub_count(n0, k, pi);
+            if (c <= L) continue;
+            const uint32_t off = noff[pi], h = c >> 1, lc = h + (c & 1);
+            const uint32_t p = chunk * 64 + lane;
             if (p < h) {
-                uint32_t* u = uf + off;
-                uf_find<GroupForest>(u, input_pair_slot<uint32_t>(p, c), s);
-                uf_find<GroupForest>(u, output_pair_slot<uint32_t>(p, c), t);
+                const uint32_t pos = off + p;
+                const bool s = ((sbits[pos >> 6] >> (pos & 63)) & 1) != 0;
                 const uint32_t a0 = perm[off + 2 * p], a1 = perm[off + 2 * p + 1];
-                const uint32_t lc = h + (c & 1);
                 next[off + p] = static_cast<uint16_t>((s ? a1 : a0) >> 1);
                 next[off + lc + p] = static_cast<uint16_t>((s ? a0 : a1) >> 1);
             } else if ((c & 1) && p == h) {
                 next[off + h] = static_cast<uint16_t>(perm[off + c - 1] >> 1);
-            }
-            const uint64_t ms = __ballot(s != 0), mt = __ballot(t != 0);
-            if (lane == 0) {
-                const uint32_t pos = off + chunk * 64, w = pos >> 6, sh = pos & 63;
-                if (ms) {
-                    atomicOr(reinterpret_cast<unsigned long long*>(sbits + w), ms << sh);
-                    if (sh) atomicOr(reinterpret_cast<unsigned long long*>(sbits + w + 1),
-                                     ms >> (64 - sh));
-                }
-                if (mt) {
-                    atomicOr(reinterpret_cast<unsigned long long*>(tbits + w), mt << sh);
-                    if (sh) atomicOr(reinterpret_cast<unsigned long long*>(tbits + w + 1),
-                                     mt >> (64 - sh));
-                }
             }
         }
         __syncthreads();

@@ -1,16 +1,17 @@
 // SPDX-License-Identifier: Apache-2.0
-// Parity union-find over the pair graph of one routing node.
+// Parity union-find over the input slots of one routing node.
 //
-// Vertices ("slots") are the node's input pairs and output pairs; entry x of
-// a node with permutation P is an edge between input pair x>>1 and output pair
-// P[x]>>1 whose label (x ^ P[x]) & 1 is the XOR of the two pairs' switch
-// values.  A parent word holds (parent slot << 1) | parity-to-parent; the
-// all-ones word marks a root.  Hooking always puts the larger root under the
-// smaller one, so every tree is rooted at its minimum slot.  Input-pair slots
-// are numbered below output-pair slots and the unpaired odd tail input gets
-// slot 0, so each component's root is the vertex the reference colours 0 first
-// (benes.cpp:278-297: the odd tail, else the component's smallest input), and a
-// slot's parity to its root is exactly its canonical switch value.
+// Slot s is input pair s (or, in an odd node, slot 0 is the unpaired tail
+// input and pair p is slot p + 1).  The two entries I[2q], I[2q + 1] reaching
+// output pair q must take different colours, which is an edge between their
+// slots labelled 1 ^ (I[2q] & 1) ^ (I[2q + 1] & 1): the XOR of the two slots'
+// switch values (a slot's colour is that of its even entry).  A parent word
+// holds (parent slot << 1) | parity-to-parent; the all-ones word marks a
+// root.  Hooking always puts the larger root under the smaller one, so every
+// tree is rooted at its minimum slot, the tail being slot 0: each component's
+// root is the vertex the reference colours 0 first (benes.cpp:278-297: the
+// odd tail, else the component's smallest input), and a slot's parity to its
+// root is exactly its canonical switch value.
 //
 // Why cached loads and plain stores are enough.  Every word ever stored is a
 // true parity relation from a slot to a strictly smaller slot: a hook links a
@@ -138,21 +139,7 @@ __device__ __forceinline__ void uf_link(W* uf, W ru, uint32_t pu, W rv, uint32_t
     }
 }
 
-// Union of input slot u and output slot v (u < v).  Output slots never get
-// children (every hook links a larger root under a smaller one, and an input
-// slot is smaller than every output slot), so when v is still a root the
-// edge is settled by hooking v straight under u.
-template <typename F, typename W>
-__device__ __forceinline__ void uf_union_edge(W* uf, W u, W v, uint32_t label) {
-    const W found = F::hook(uf + v, (u << 1) | static_cast<W>(label & 1));
-    if (found == UfWord<W>::root) return;
-    uint32_t pu, pv;
-    const W ru = uf_find<F>(uf, u, pu);
-    const W rv = uf_find<F>(uf, found >> 1, pv);
-    uf_link<F>(uf, ru, pu, rv, pv ^ static_cast<uint32_t>(found & 1), label);
-}
-
-// Union of two input slots (the input-pair forest used on HBM levels).
+// Union of two input slots.
 template <typename F, typename W>
 __device__ __forceinline__ void uf_union_slots(W* uf, W a, W b, uint32_t label) {
     uint32_t pa, pb;
@@ -171,11 +158,6 @@ __device__ __forceinline__ W input_slot(uint64_t x, uint64_t count) {
 template <typename W>
 __device__ __forceinline__ W input_pair_slot(uint64_t pair, uint64_t count) {
     return static_cast<W>(pair + (count & 1));
-}
-
-template <typename W>
-__device__ __forceinline__ W output_pair_slot(uint64_t pair, uint64_t count) {
-    return static_cast<W>((count >> 1) + (count & 1) + pair);
 }
 
 }  // namespace benes_hip
