@@ -28,7 +28,7 @@ LIB_PATH = os.path.join(HERE, "libbenes_hip.so")
 OK, INVALID_ARGUMENT, OVERFLOW, LENGTH, LOGIC, RUNTIME, BUFFER_TOO_SMALL, PENDING = range(8)
 
 SEPARATE, EMBEDDED_INPUT, EMBEDDED_OUTPUT = 0, 1, 2
-FISHER_YATES, SINGLE_CYCLE, NEAR_IDENTITY, IDENTITY, REVERSE = range(5)
+FISHER_YATES, SINGLE_CYCLE, NEAR_IDENTITY, IDENTITY, REVERSE, KEYED = range(6)
 
 # Every extern "C" symbol declared in include/benes_hip.h.
 EXPORTS = (
@@ -43,6 +43,8 @@ EXPORTS = (
     "benes_hip_server_wait",
     "benes_hip_server_stats",
     "benes_hip_server_destroy",
+    "benes_hip_generate_device",
+    "benes_hip_release_workspace",
     "benes_hip_encoded_size",
     "benes_hip_last_device_seconds",
     "benes_hip_set_profiling",
@@ -166,6 +168,8 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_address_bit.argtypes = [ctypes.POINTER(_Address), u64]
     lib.benes_hip_address_bit.restype = u64
     lib.benes_hip_generate.argtypes = [u32, u64, u64, u32, vp, u32]
+    lib.benes_hip_generate_device.argtypes = [u32, u64, u64, vp, u32, vp]
+    lib.benes_hip_release_workspace.restype = None
     _lib = lib
     return lib
 
@@ -430,3 +434,17 @@ def generate(kind: int, n: int, seed: int, rate_ppm: int = 0, width: int = 4) ->
     if rc != OK:
         raise ValueError("invalid generator arguments")
     return out
+
+
+def generate_device(kind: int, n: int, seed: int, d_out_ptr: int, width: int = 8,
+                    stream: int | None = None) -> None:
+    """IDENTITY, REVERSE or KEYED permutation written into device memory."""
+    rc = library().benes_hip_generate_device(kind, n, seed, ctypes.c_void_p(d_out_ptr), width,
+                                             ctypes.c_void_p(stream or 0))
+    if rc != OK:
+        raise ValueError("invalid device generator arguments")
+
+
+def release_workspace() -> None:
+    """Frees the library's device buffers for the current device."""
+    library().benes_hip_release_workspace()

@@ -22,12 +22,14 @@
 #include <unordered_map>
 #include <cstring>
 #include <mutex>
+#include <stdexcept>
 #include <string>
 #include <type_traits>
 #include <vector>
 
 #include "../../include/benes_hip.h"
 #include "host_runtime.hpp"
+#include "keyed.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 #include "status.hpp"
@@ -60,8 +62,11 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() {
+    ~DevBuf() { release(); }
+    void release() {
         if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
     }
     void ensure(size_t need) {
         if (need <= bytes) return;
@@ -280,11 +285,13 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         write_header(jobs[i].lay, items[i].header);
     }
 
-    // Segment tables of every launch.
+    // Segment tables of every launch.  Per-entry kernels cover 2^31 entries
+    // per pass at most and loop over the rest.
+    const uint64_t per_thread = std::max<uint64_t>(1, ceil_div(total_n, uint64_t{1} << 31));
     std::vector<Seg> table;
     SegList per_entry_list;
     for (uint32_t i = 0; i < count; ++i) {
-        per_entry_list.add(i, ceil_div(jobs[i].n, kBlock), 0, 0, 0, 0, 0, false);
+        per_entry_list.add(i, ceil_div(jobs[i].n, kBlock * per_thread), 0, 0, 0, 0, 0, false);
     }
     const Launch per_entry = per_entry_list.commit(table);
     struct LevelPlan {
@@ -308,7 +315,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             if (cmax > kLevelLds) {
                 const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
                 hbm.add(i, nodes * chunks, nodes, chunks, desc_count, row_count, row_words, true);
-                fill.add(i, ceil_div(jobs[i].n, kBlock), 0, 0, 0, 0, 0, false);
+                fill.add(i, ceil_div(jobs[i].n, kBlock * per_thread), 0, 0, 0, 0, 0, false);
                 lp.wide_uf = lp.wide_uf || maxpairs + 1 >= (uint64_t{1} << 31);
             } else {
                 lds.add(i, nodes, nodes, 1, desc_count, row_count, row_words, false);
@@ -339,6 +346,15 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         subtree_list.add(i, uint64_t{1} << jobs[i].df, 0, 0, 0, 0, 0, false);
     }
     const Launch subtree = subtree_list.commit(table);
+    // One dispatch holds at most 2^32 - 1 work-items.
+    uint64_t most = subtree.blocks * kSubThreads;
+    for (const LevelPlan& lp : levels) {
+        most = std::max({most, lp.hbm.blocks * kBlock, lp.lds.blocks * kLevelThreads,
+                         lp.pack[0].blocks * kBlock, lp.pack[1].blocks * kBlock});
+    }
+    if (most >= (uint64_t{1} << 32)) {
+        throw std::length_error("batch too large for one call: split it into smaller batches");
+    }
 
     // Scratch.
     ws.segs.ensure(sizeof(Seg) * std::max<size_t>(1, table.size()));
@@ -472,6 +488,10 @@ bool check_input(uint64_t n, uint32_t width, Status& st) {
         st.set(kInvalidArgument, "permutation size requires 64-bit source values");
         return false;
     }
+    if (n > (uint64_t{1} << 33)) {  // beyond one device's memory and dispatch limits
+        st.set(kLength, "permutation too large for one device (at most 2^33 entries)");
+        return false;
+    }
     return true;
 }
 
@@ -509,6 +529,20 @@ int current_device(Status& st) {
         return -1;
     }
     return dev;
+}
+
+// Device generation of the permutations computable entry by entry.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_generate(uint32_t kind, uint64_t n, uint64_t seed,
+                                                     T* __restrict__ out) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += stride) {
+        uint64_t v = i;
+        if (kind == BENES_HIP_PERM_REVERSE) v = n - 1 - i;
+        else if (kind == BENES_HIP_PERM_KEYED) v = keyed_perm(i, n, seed);
+        out[i] = static_cast<T>(v);
+    }
 }
 
 const char* kNotPermutation = "input must contain every value from zero through size minus one";
@@ -565,6 +599,8 @@ int with_workspace(Status& st, Body&& body) {
         body(ws);
     } catch (const HipError& e) {
         st.set(kRuntime, std::string("hip error: ") + hipGetErrorString(e.code) + " in " + e.what);
+    } catch (const std::length_error& e) {
+        st.set(kLength, e.what());
     } catch (const std::bad_alloc&) {
         st.set(kLength, "allocation failed");
     }
@@ -792,6 +828,8 @@ void Server::serve(Lane& lane) {
             process(lane);
         } catch (const HipError& e) {
             error = std::string("hip error: ") + hipGetErrorString(e.code) + " in " + e.what;
+        } catch (const std::length_error& e) {
+            error = e.what();
         } catch (const std::bad_alloc&) {
             error = "allocation failed";
         }
@@ -962,6 +1000,45 @@ int benes_hip_compress_batch_device(benes_hip_item* items, uint64_t count, uint8
         }
     });
     return finish(st, err, err_len);
+}
+
+int benes_hip_generate_device(uint32_t kind, uint64_t n, uint64_t seed, void* d_out,
+                              uint32_t width_bytes, void* stream) {
+    if (kind != BENES_HIP_PERM_IDENTITY && kind != BENES_HIP_PERM_REVERSE &&
+        kind != BENES_HIP_PERM_KEYED) {
+        return BENES_HIP_INVALID_ARGUMENT;
+    }
+    if ((width_bytes != 4 && width_bytes != 8) || (width_bytes == 4 && n > 0x100000000ull) ||
+        (n && !d_out)) {
+        return BENES_HIP_INVALID_ARGUMENT;
+    }
+    if (n == 0) return BENES_HIP_OK;
+    const auto s = static_cast<hipStream_t>(stream);
+    const uint64_t blocks = std::min<uint64_t>(ceil_div(n, kBlock), uint64_t{1} << 20);
+    if (width_bytes == 4) {
+        hipLaunchKernelGGL(k_generate<uint32_t>, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock),
+                           0, s, kind, n, seed, static_cast<uint32_t*>(d_out));
+    } else {
+        hipLaunchKernelGGL(k_generate<uint64_t>, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock),
+                           0, s, kind, n, seed, static_cast<uint64_t*>(d_out));
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        return BENES_HIP_RUNTIME;
+    }
+    return BENES_HIP_OK;
+}
+
+void benes_hip_release_workspace(void) {
+    int dev = -1;
+    if (!benes_hip_available() || hipGetDevice(&dev) != hipSuccess) return;
+    Workspace& ws = workspace_for(dev);
+    std::lock_guard<std::mutex> lock(ws.mu);
+    if (ws.stream) (void)hipStreamSynchronize(ws.stream);
+    for (DevBuf* b : {&ws.input, &ws.level[0], &ws.level[1], &ws.ilevel[0], &ws.ilevel[1],
+                      &ws.inv0, &ws.uf, &ws.srows, &ws.trows, &ws.desc, &ws.plans, &ws.items,
+                      &ws.segs, &ws.errs, &ws.blob}) {
+        b->release();
+    }
 }
 
 int benes_hip_encoded_size(uint64_t n, const benes_hip_format* format, uint64_t* out_size,

@@ -152,6 +152,8 @@ __device__ __forceinline__ uint32_t row_bit(const uint64_t* rows, uint64_t row, 
 // Input validation and the inverse (benes.cpp:152-173).
 //
 // inv starts as all-ones; every in-range entry stores its index at its value.
+// Per-entry kernels stride over their item with the blocks of its segment, so
+// a launch stays within 2^32 work-items whatever the batch holds.
 // A value out of range raises the item's error flag; a duplicate leaves some
 // other slot unwritten, which k_check_inverse reports.  No atomics: on this
 // device every global atomic costs several random stores.
@@ -163,14 +165,15 @@ __global__ __launch_bounds__(kBlock) void k_scatter_inverse(const ItemCtx* __res
                                                             int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     const ItemCtx it = items[sg.item];
-    const uint64_t i = (blockIdx.x - sg.block0) * kBlock + threadIdx.x;
-    if (i >= it.n) return;
-    const uint64_t v = load_src(it, i);
-    if (v >= it.n) {
-        errs[sg.item] = 1;
-        return;
+    for (uint64_t i = (blockIdx.x - sg.block0) * kBlock + threadIdx.x; i < it.n;
+         i += sg.blocks * kBlock) {
+        const uint64_t v = load_src(it, i);
+        if (v >= it.n) {
+            errs[sg.item] = 1;
+            return;
+        }
+        inv[it.base + v] = static_cast<VI>(i);
     }
-    inv[it.base + v] = static_cast<VI>(i);
 }
 
 // A slot of the inverse still all-ones was reached by no entry: some value
@@ -182,8 +185,10 @@ __global__ __launch_bounds__(kBlock) void k_check_inverse(const ItemCtx* __restr
                                                           int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     const ItemCtx it = items[sg.item];
-    const uint64_t i = (blockIdx.x - sg.block0) * kBlock + threadIdx.x;
-    if (i < it.n && inv[it.base + i] == static_cast<VI>(~VI{0})) errs[sg.item] = 1;
+    for (uint64_t i = (blockIdx.x - sg.block0) * kBlock + threadIdx.x; i < it.n;
+         i += sg.blocks * kBlock) {
+        if (inv[it.base + i] == static_cast<VI>(~VI{0})) errs[sg.item] = 1;
+    }
 }
 
 // Sets words [base, base + n) of each item to all-ones (union-find roots).
@@ -193,8 +198,10 @@ __global__ __launch_bounds__(kBlock) void k_fill_roots(const ItemCtx* __restrict
                                                        uint32_t nsegs, W* __restrict__ uf) {
     const Seg& sg = find_seg(segs, nsegs);
     const ItemCtx it = items[sg.item];
-    const uint64_t i = (blockIdx.x - sg.block0) * kBlock + threadIdx.x;
-    if (i < it.n) uf[it.base + i] = UfWord<W>::root;
+    for (uint64_t i = (blockIdx.x - sg.block0) * kBlock + threadIdx.x; i < it.n;
+         i += sg.blocks * kBlock) {
+        uf[it.base + i] = UfWord<W>::root;
+    }
 }
 
 // Writes each item's header bytes (internal.hpp:431-454) at the start of its

@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "../../include/benes_hip.h"
+#include "keyed.hpp"
 
 namespace {
 
@@ -38,6 +39,10 @@ struct SplitMix64 {
 template <typename T>
 void fill(uint32_t kind, uint64_t n, uint64_t seed, uint32_t rate_ppm, T* out) {
     SplitMix64 rng{seed};
+    if (kind == BENES_HIP_PERM_KEYED) {
+        for (uint64_t i = 0; i < n; ++i) out[i] = static_cast<T>(benes_hip::keyed_perm(i, n, seed));
+        return;
+    }
     if (kind == BENES_HIP_PERM_REVERSE) {
         for (uint64_t i = 0; i < n; ++i) out[i] = static_cast<T>(n - 1 - i);
         return;
@@ -75,7 +80,7 @@ void fill(uint32_t kind, uint64_t n, uint64_t seed, uint32_t rate_ppm, T* out) {
 
 extern "C" int benes_hip_generate(uint32_t kind, uint64_t n, uint64_t seed, uint32_t rate_ppm,
                                   void* out, uint32_t width_bytes) {
-    if (kind > BENES_HIP_PERM_REVERSE) return BENES_HIP_INVALID_ARGUMENT;
+    if (kind > BENES_HIP_PERM_KEYED) return BENES_HIP_INVALID_ARGUMENT;
     if (width_bytes == 4) {
         if (n > 0x100000000ull) return BENES_HIP_INVALID_ARGUMENT;
         fill(kind, n, seed, rate_ppm, static_cast<uint32_t*>(out));

@@ -214,12 +214,26 @@ uint64_t benes_hip_address_bit(const benes_hip_address* address, uint64_t pair);
 #define BENES_HIP_PERM_NEAR_IDENTITY 2 /* identity + floor(rate_ppm*n/1e6) swaps */
 #define BENES_HIP_PERM_IDENTITY 3
 #define BENES_HIP_PERM_REVERSE 4
+#define BENES_HIP_PERM_KEYED 5 /* keyed Feistel bijection walked into [0, n); parallel */
 
 /* Writes n entries of width_bytes (4 or 8) into `out` (host memory).  The
  * generator is splitmix64 with Lemire's bounded draw, so the sequence is fixed
  * by (kind, n, seed, rate_ppm) on every platform. */
 int benes_hip_generate(uint32_t kind, uint64_t n, uint64_t seed, uint32_t rate_ppm,
                        void* out, uint32_t width_bytes);
+
+/* The same permutations written straight into device memory `d_out` on
+ * `stream` (returns once they are written), for the kinds computable entry by
+ * entry: IDENTITY, REVERSE and KEYED (the others are sequential draws and
+ * return BENES_HIP_INVALID_ARGUMENT).  Used for inputs too large to generate
+ * on the host, such as 2^32 + 12345 entries. */
+int benes_hip_generate_device(uint32_t kind, uint64_t n, uint64_t seed, void* d_out,
+                              uint32_t width_bytes, void* stream);
+
+/* Frees the device buffers the library keeps for the calling thread's
+ * current device (they grow to the largest call so far and are otherwise
+ * kept for reuse).  No counterpart in the reference. */
+void benes_hip_release_workspace(void);
 
 #ifdef __cplusplus
 }

@@ -909,6 +909,25 @@ int orc_lookup(const uint8_t* blob, uint64_t size, uint64_t value, int inverse,
     return report(&st, err, err_len);
 }
 
+/* Forward (or inverse) images of `count` values, parsing the blob once: the
+ * sampled round trip used where a whole decode would be too long. */
+int orc_lookup_many(const uint8_t* blob, uint64_t size, const uint64_t* values,
+                    uint64_t count, int inverse, uint64_t* results, char* err,
+                    uint64_t err_len) {
+    Status st = {0, {0}};
+    orc_layout L;
+    parse_layout(&st, blob, size, &L);
+    if (!st.status) {
+        Sizes* s = malloc(sizeof(Sizes));
+        sizes_init(s, L.middle_group_log2, &st);
+        for (uint64_t i = 0; i < count && !st.status; ++i) {
+            results[i] = lookup(&st, blob, &L, s, values[i], inverse);
+        }
+        free(s);
+    }
+    return report(&st, err, err_len);
+}
+
 /* Decodes every forward image (and, when `inverse_out` is non-null, every
  * inverse image) of a blob: the round-trip verifier used by the tests. */
 int orc_decode_all(const uint8_t* blob, uint64_t size, uint64_t* forward_out,

@@ -81,6 +81,10 @@ int orc_compress(const void* perm, uint64_t n, uint32_t width_bytes,
 int orc_lookup(const uint8_t* blob, uint64_t size, uint64_t value,
                int inverse, uint64_t* result, char* err, uint64_t err_len);
 
+int orc_lookup_many(const uint8_t* blob, uint64_t size, const uint64_t* values,
+                    uint64_t count, int inverse, uint64_t* results, char* err,
+                    uint64_t err_len);
+
 int orc_decode_all(const uint8_t* blob, uint64_t size, uint64_t* forward_out,
                    uint64_t* inverse_out, char* err, uint64_t err_len);
 

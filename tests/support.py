@@ -67,6 +67,7 @@ def oracle() -> ctypes.CDLL:
         lib.orc_lookup.argtypes = [vp, u64, u64, ctypes.c_int, ctypes.POINTER(u64),
                                    ctypes.c_char_p, u64]
         lib.orc_decode_all.argtypes = [vp, u64, vp, vp, ctypes.c_char_p, u64]
+        lib.orc_lookup_many.argtypes = [vp, u64, vp, u64, ctypes.c_int, vp, ctypes.c_char_p, u64]
         lib.orc_node_address.argtypes = [ctypes.POINTER(OrcLayout), ctypes.c_int,
                                          ctypes.c_uint32, u64, ctypes.POINTER(OrcAddress)]
         lib.orc_factoradic_bits.argtypes = [ctypes.c_uint32]
@@ -161,6 +162,21 @@ def oracle_decode(blob: bytes, inverse: bool = False) -> np.ndarray:
     if rc:
         raise _EXC.get(rc, RuntimeError)(err.value.decode())
     return (fwd, inv) if inverse else fwd
+
+
+def oracle_lookup(blob: np.ndarray, values, inverse: bool = False) -> np.ndarray:
+    """Images of `values` under the blob's permutation (or its inverse),
+    decoded by the oracle; `blob` is a uint8 array holding the whole blob."""
+    vals = np.ascontiguousarray(values, np.uint64)
+    res = np.zeros(vals.size, np.uint64)
+    err = ctypes.create_string_buffer(256)
+    rc = oracle().orc_lookup_many(blob.ctypes.data_as(ctypes.c_void_p), blob.size,
+                                  vals.ctypes.data_as(ctypes.c_void_p), vals.size,
+                                  1 if inverse else 0, res.ctypes.data_as(ctypes.c_void_p),
+                                  err, 256)
+    if rc:
+        raise _EXC.get(rc, RuntimeError)(err.value.decode())
+    return res
 
 
 def oracle_layout(n, g=5, placement=0, ic=(), oc=()) -> OrcLayout:

@@ -128,3 +128,14 @@ def test_generators_are_fixed():
     # near-identity moves at most 2 * floor(rate * n) entries
     p = B.generate(B.NEAR_IDENTITY, 100000, 3, 5000)
     assert (p != np.arange(100000)).sum() <= 2 * 500
+
+
+def test_keyed_generator_is_a_bijection():
+    for n in (1, 2, 3, 5, 17, 1000, 65537, (1 << 20) + 3):
+        for width in (4, 8):
+            p = B.generate(B.KEYED, n, 77, 0, width)
+            assert np.array_equal(np.sort(p), np.arange(n, dtype=p.dtype))
+            assert np.array_equal(p, B.generate(B.KEYED, n, 77, 0, width))
+    a, b = B.generate(B.KEYED, 5000, 1), B.generate(B.KEYED, 5000, 2)
+    assert not np.array_equal(a, b)
+    assert (a != np.arange(5000)).sum() > 4900   # no structure left to speak of

@@ -1,0 +1,139 @@
+# SPDX-License-Identifier: Apache-2.0
+"""The 64-bit path at its full size: N = 2^32 + 12345 uint64 entries,
+middle_group_log2 5, default clusters, ranks embedded in the output section.
+
+At this size the root has more than 2^31 input slots and the inverse needs
+64-bit words, so the wide union-find and inverse variants run, along with the
+pruned ceil/floor recursion and the short final groups.  An oracle encode at
+this size is out of reach, so the blobs are checked through properties that
+do not depend on size:
+  * reverse: a reversal routes into two reversals; every input switch and
+    every rank is 0, and the output switches are 1 exactly at nodes of even
+    size (the closed form the oracle tests pin against the reference's
+    reverse-33 golden blob and the oracle's own encoder), so the input section
+    is all zero and the output section holds exactly the even nodes' pairs as
+    ones;
+  * identity: every switch bit is 0 and every leaf rank is c! - 1, so the
+    input section is all zero, the output section's ones are exactly those of
+    the ranks, and sampled entries decode to themselves;
+  * a keyed pseudo-random bijection: sampled entries decode, forward and
+    backward, to the values that were compressed.
+Inputs are generated on the device; the permutation (34 GB) never crosses to
+the host, the blob (16.5 GB) does once per case for the oracle's decoder.
+"""
+from __future__ import annotations
+
+import functools
+import math
+
+import numpy as np
+import pytest
+
+import support as S
+
+B = S.benes_hip
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+N = (1 << 32) + 12345
+FMT = S.fmt(5, 2)          # default clusters {14,14}/{14,14}, embedded_output
+SAMPLES = 3000
+
+
+@pytest.fixture(scope="module")
+def big():
+    assert B.available(), "no gfx950 device or libbenes_hip.so failed to load its kernels"
+    free, _ = torch.cuda.mem_get_info()
+    if free < (200 << 30):
+        pytest.skip(f"needs about 200 GB of free device memory, {free >> 30} GB free")
+    size = B.encoded_size(N, FMT)
+    d_perm = torch.empty(N, dtype=torch.int64, device="cuda")
+    d_out = torch.empty((size + 7) // 8 * 8, dtype=torch.uint8, device="cuda")
+    lay = B.layout_of(N, FMT)
+    yield d_perm, d_out, size, lay
+    del d_perm, d_out
+    torch.cuda.empty_cache()
+    B.release_workspace()
+
+
+LEAF = 32
+
+
+@functools.lru_cache(maxsize=None)
+def even_pairs(c: int) -> int:
+    """Output switches set by a reversal of c entries (pairs of even nodes)."""
+    if c <= LEAF:
+        return 0
+    return (c // 2 if c % 2 == 0 else 0) + even_pairs((c + 1) // 2) + even_pairs(c // 2)
+
+
+@functools.lru_cache(maxsize=None)
+def identity_rank_ones(c: int) -> int:
+    """Ones in the ranks of an identity of c entries (c! - 1 at every leaf)."""
+    if c <= LEAF:
+        return bin(math.factorial(c) - 1).count("1")
+    return identity_rank_ones((c + 1) // 2) + identity_rank_ones(c // 2)
+
+
+def ones(t) -> int:
+    """Set bits of a uint8 device tensor, counted in slices."""
+    total = 0
+    step = 256 << 20
+    for a in range(0, t.numel(), step):
+        x = t[a:a + step].to(torch.int32)
+        for k in range(8):
+            total += int(((x >> k) & 1).sum())
+    return total
+
+
+def run(big, kind, seed=0):
+    d_perm, d_out, size, _ = big
+    B.generate_device(kind, N, seed, d_perm.data_ptr(), 8)
+    got = B.compress_device(d_perm.data_ptr(), N, 8, d_out.data_ptr(), d_out.numel(), FMT)
+    torch.cuda.synchronize()
+    assert got == size
+    return d_out[:size]
+
+
+def sample_indices(seed):
+    rng = np.random.default_rng(seed)
+    edges = np.array([0, 1, 2, N // 2, N - 3, N - 2, N - 1], np.uint64)
+    return np.concatenate([edges, rng.integers(0, N, SAMPLES, dtype=np.uint64)])
+
+
+def test_generators_agree_with_host():
+    n = (1 << 20) + 7
+    d = torch.empty(n, dtype=torch.int64, device="cuda")
+    for kind in (B.IDENTITY, B.REVERSE, B.KEYED):
+        B.generate_device(kind, n, 99, d.data_ptr(), 8)
+        assert np.array_equal(d.cpu().numpy().view(np.uint64), B.generate(kind, n, 99, 0, 8))
+
+
+def test_reverse_closed_form(big):
+    blob = run(big, B.REVERSE)
+    lay = big[3]
+    assert int(torch.count_nonzero(blob[lay["input_offset"]:lay["output_offset"]])) == 0
+    assert ones(blob[lay["output_offset"]:]) == even_pairs(N)
+    host = blob[:4096].cpu().numpy()
+    assert int.from_bytes(host[8:16].tobytes(), "little") == N   # entry count in the header
+
+
+def test_identity(big):
+    blob = run(big, B.IDENTITY)
+    lay = big[3]
+    assert int(torch.count_nonzero(blob[lay["input_offset"]:lay["output_offset"]])) == 0
+    assert ones(blob[lay["output_offset"]:]) == identity_rank_ones(N)
+    host = blob.cpu().numpy()
+    idx = sample_indices(1)
+    assert np.array_equal(S.oracle_lookup(host, idx), idx)
+    assert np.array_equal(S.oracle_lookup(host, idx, inverse=True), idx)
+
+
+def test_keyed_round_trip(big):
+    blob = run(big, B.KEYED, seed=0x5EED)
+    d_perm = big[0]
+    host = blob.cpu().numpy()
+    idx = sample_indices(2)
+    want = d_perm[torch.from_numpy(idx.astype(np.int64)).cuda()].cpu().numpy().view(np.uint64)
+    assert np.array_equal(S.oracle_lookup(host, idx), want)
+    assert np.array_equal(S.oracle_lookup(host, want, inverse=True), idx)
