@@ -215,7 +215,8 @@ struct SegList {
     std::vector<Seg> segs;
     uint64_t blocks = 0;
     void add(uint32_t item, uint64_t blocks_of_item, uint64_t nodes, uint64_t chunks,
-             uint64_t desc0, uint64_t row0, uint64_t row_words, bool xcd_align) {
+             uint64_t desc0, uint64_t row0, uint64_t row_words, bool xcd_align,
+             bool by_lane = false) {
         if (blocks_of_item == 0) return;
         if (xcd_align) {
             blocks = round8(blocks);
@@ -230,6 +231,7 @@ struct SegList {
         s.row0 = row0;
         s.row_words = row_words;
         s.item = item;
+        s.lanes = by_lane ? 1u : 0u;
         segs.push_back(s);
         blocks += blocks_of_item;
     }
@@ -326,8 +328,11 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                 const uint32_t span_log2 = sd.group_depth - sd.relative_level - 1;
                 const uint64_t packets = ceil_div(maxpairs, uint64_t{1} << span_log2);
                 const uint64_t groups = uint64_t{1} << (d - sd.relative_level);
-                pack[side].add(i, ceil_div(groups * packets, kBlock / 64), nodes, packets,
-                               desc_count, row_count, row_words, false);
+                // A lane per job when a full packet's share of the level fits
+                // one word (2^(group depth - 1) bits), else a wave per job.
+                const bool by_lane = sd.group_depth <= 7;
+                pack[side].add(i, ceil_div(groups * packets, by_lane ? kBlock : kBlock / 64),
+                               nodes, packets, desc_count, row_count, row_words, false, by_lane);
             }
             desc_count += nodes;
             row_count += nodes * row_words;
@@ -791,14 +796,24 @@ void Server::process(Lane& lane) {
         at += round8(len);
     }
     const hipStream_t s = lane.ws.stream;
+    auto t0 = std::chrono::steady_clock::now();
     lane.ws.ring.upload(lane.ws.input.as<uint8_t>(), in, s);
+    note_host_time("host:serve_upload", t0);
     Profiler prof;
+    {
+        std::lock_guard<std::mutex> plock(g_prof_mu);
+        prof.on = g_profiling;
+    }
     prof.stream = s;
+    t0 = std::chrono::steady_clock::now();
     enqueue_batch(lane.ws, lane.jobs, prof, nullptr);
+    note_host_time("host:serve_enqueue", t0);
     std::vector<int> flags(batch.size());
     BH_CHECK(hipMemcpyAsync(flags.data(), lane.ws.errs.ptr, sizeof(int) * batch.size(),
                             hipMemcpyDeviceToHost, s));
     BH_CHECK(hipStreamSynchronize(s));
+    note_host_time("host:serve_device", t0);
+    prof.collect();
     std::vector<HostPiece> out;
     for (size_t k = 0; k < batch.size(); ++k) {
         Request* r = batch[k];
@@ -814,7 +829,9 @@ void Server::process(Lane& lane) {
             r->status = kOk;
         }
     }
+    t0 = std::chrono::steady_clock::now();
     lane.ws.ring.download(lane.ws.blob.as<uint8_t>(), out, s);
+    note_host_time("host:serve_download", t0);
 }
 
 void Server::serve(Lane& lane) {

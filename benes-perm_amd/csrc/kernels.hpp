@@ -72,7 +72,7 @@ struct Seg {
     uint64_t row0;       // first row word of the item at this depth
     uint64_t row_words;  // row words per node
     uint32_t item;
-    uint32_t pad;
+    uint32_t lanes;      // k_pack: one lane (1) or one wave (0) per job
 };
 
 // Segment of the block: the last one whose first block is <= blockIdx.x.
@@ -98,24 +98,62 @@ __device__ __forceinline__ uint64_t load_src(const ItemCtx& it, uint64_t i) {
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63; }
 
-// ORs one bit per lane into the blob.  Addresses must be strictly increasing
-// over the valid lanes, which then form a prefix of the wave.  Lanes hitting
-// the same 64-bit word are merged first, so each word takes one atomic.
-__device__ __forceinline__ void wave_or_bits(uint64_t* blob, uint64_t addr, bool bit,
-                                             bool valid) {
+// Inclusive prefix sum over the lanes of a wave.
+__device__ __forceinline__ uint64_t wave_inclusive_sum(uint64_t v) {
     const uint32_t lane = lane_id();
-    const uint64_t key = valid ? (addr >> 6) : ~uint64_t{0};
-    uint64_t val = (valid && bit) ? (uint64_t{1} << (addr & 63)) : 0;
 #pragma unroll
     for (uint32_t off = 1; off < 64; off <<= 1) {
-        const uint64_t ov = __shfl_down(val, off);
-        const uint64_t ok = __shfl_down(key, off);
-        if (lane + off < 64 && ok == key) val |= ov;
+        const uint64_t o = __shfl_up(v, off);
+        if (lane >= off) v += o;
     }
-    const uint64_t prev = __shfl_up(key, 1);
-    const bool head = lane == 0 || prev != key;
-    if (head && valid && val) atomicOr(reinterpret_cast<unsigned long long*>(blob + key),
-                                       static_cast<unsigned long long>(val));
+    return v;
+}
+
+// ORs `len` bits, bit b given by get(b), into blob bits [dst, dst + len) from
+// one lane: 64 bits at a time, each with at most two atomics (neighbouring
+// runs of other lanes share the edge words).
+template <typename Get>
+__device__ __forceinline__ void lane_or_bits(uint64_t* blob, uint64_t dst, uint64_t len, Get get) {
+    for (uint64_t b0 = 0; b0 < len; b0 += 64) {
+        const uint32_t cnt = static_cast<uint32_t>(min(uint64_t{64}, len - b0));
+        uint64_t acc = 0;
+        for (uint32_t b = 0; b < cnt; ++b) acc |= static_cast<uint64_t>(get(b0 + b) ? 1 : 0) << b;
+        if (acc == 0) continue;
+        const uint64_t at = dst + b0;
+        auto* w = reinterpret_cast<unsigned long long*>(blob) + (at >> 6);
+        const uint32_t sh = at & 63;
+        atomicOr(w, static_cast<unsigned long long>(acc << sh));
+        if (sh && (acc >> (64 - sh))) atomicOr(w + 1, static_cast<unsigned long long>(acc >> (64 - sh)));
+    }
+}
+
+// ORs the low `len` (<= 64) bits of `v` into blob bits [dst, dst + len).
+__device__ __forceinline__ void or_word(uint64_t* blob, uint64_t dst, uint64_t v) {
+    if (v == 0) return;
+    auto* w = reinterpret_cast<unsigned long long*>(blob) + (dst >> 6);
+    const uint32_t sh = dst & 63;
+    atomicOr(w, static_cast<unsigned long long>(v << sh));
+    if (sh && (v >> (64 - sh))) atomicOr(w + 1, static_cast<unsigned long long>(v >> (64 - sh)));
+}
+
+// Bits [first, first + len) (len <= 64) of the bit row starting at word `row`.
+__device__ __forceinline__ uint64_t row_range(const uint64_t* rows, uint64_t row, uint64_t first,
+                                              uint32_t len) {
+    const uint64_t w = row + (first >> 6);
+    const uint32_t sh = first & 63;
+    uint64_t v = rows[w] >> sh;
+    if (sh && sh + len > 64) v |= rows[w + 1] << (64 - sh);
+    return len >= 64 ? v : v & ((uint64_t{1} << len) - 1);
+}
+
+// ORs bits [first, first + len) of a bit row into blob bits [dst, dst + len)
+// from one lane, a word at a time.
+__device__ __forceinline__ void lane_or_row(uint64_t* blob, uint64_t dst, const uint64_t* rows,
+                                            uint64_t row, uint64_t first, uint64_t len) {
+    for (uint64_t b = 0; b < len; b += 64) {
+        const uint32_t cnt = static_cast<uint32_t>(min(uint64_t{64}, len - b));
+        or_word(blob, dst + b, row_range(rows, row, first + b, cnt));
+    }
 }
 
 // Copies `len` bits, bit k given by get(k), to blob bits [dst0, dst0 + len).
@@ -449,7 +487,10 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
     }
 }
 
-// One wave per (cluster group, packet) of one item at depth d on one side.
+// One job per (cluster group, packet) of one item at depth d on one side:
+// a wave per job, or a lane per job when a full packet's share of the level
+// (2^(group depth - 1) bits) fits one word.  In the packet where node shares
+// differ, lanes take the nodes, each placing its share from its own address.
 __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ items,
                                                  const Seg* __restrict__ segs, uint32_t nsegs,
                                                  int side, uint32_t d,
@@ -464,38 +505,56 @@ __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ ite
     uint64_t* blob = blob_all + it.blob;
     const NodeDesc* dsc = desc + sg.desc0;
     const uint64_t packets = sg.chunks;
-    const uint64_t gw = (blockIdx.x - sg.block0) * (kBlock / 64) + (threadIdx.x >> 6);
+    const bool by_lane = sg.lanes != 0;
+    const uint64_t job = by_lane ? (blockIdx.x - sg.block0) * kBlock + threadIdx.x
+                                 : (blockIdx.x - sg.block0) * (kBlock / 64) + (threadIdx.x >> 6);
     const SideDepth sd = P.side[side][d];
     const uint32_t r = sd.relative_level, g = sd.group_depth;
     const uint32_t root_depth = d - r;
     const uint64_t groups = uint64_t{1} << root_depth;
-    const uint64_t group = gw / packets, packet = gw % packets;
-    if (group >= groups) return;
+    const uint64_t group = job / packets, packet = job % packets;
+    if (group >= groups) return;  // whole waves in wave mode; lanes in lane mode
     const uint32_t span_log2 = g - r - 1;
     const uint64_t C = node_count(P.n, root_depth, group);
     const uint64_t first_pair = packet << span_log2;
     const uint64_t min_pairs = (C >> r) >> 1;
     const uint64_t b0 = group << r;
+    const uint64_t nodes = uint64_t{1} << r;
     const uint64_t secbase = (side == 0 ? P.input_offset : P.output_offset) * 8;
-    if (min_pairs >= first_pair + (uint64_t{1} << span_log2)) {
+    const uint64_t mask = (uint64_t{1} << span_log2) - 1;
+    const bool full = min_pairs >= first_pair + (uint64_t{1} << span_log2);
+    if (by_lane) {
+        if (full) {
+            // nodes << span_log2 <= 64 bits: node j's span of pairs at bit j * span.
+            uint64_t acc = 0;
+            for (uint64_t pi = 0; pi < nodes; ++pi) {
+                acc |= row_range(rows, dsc[b0 + pi].row, first_pair, 1u << span_log2)
+                       << (pi << span_log2);
+            }
+            or_word(blob, secbase + bit_of(dsc[b0].a[side], first_pair), acc);
+        } else {
+            for (uint64_t pi = 0; pi < nodes; ++pi) {
+                const NodeDesc& nd = dsc[b0 + pi];
+                lane_or_row(blob, secbase + bit_of(nd.a[side], first_pair), rows, nd.row,
+                            first_pair, packet_share(nd.cnt, packet, span_log2, P.leaf));
+            }
+        }
+        return;
+    }
+    if (full) {
         // Every node of the level fills its share of this packet: one run.
-        const uint64_t start = bit_of(dsc[b0].a[side], first_pair);
-        const uint64_t len = uint64_t{1} << (r + span_log2);
-        const uint64_t mask = (uint64_t{1} << span_log2) - 1;
-        wave_write_run(blob, secbase + start, len, [&](uint64_t k) {
-            const NodeDesc& nd = dsc[b0 + (k >> span_log2)];
-            return row_bit(rows, nd.row, first_pair + (k & mask)) != 0;
-        });
+        wave_write_run(blob, secbase + bit_of(dsc[b0].a[side], first_pair), nodes << span_log2,
+                       [&](uint64_t k) {
+                           const NodeDesc& nd = dsc[b0 + (k >> span_log2)];
+                           return row_bit(rows, nd.row, first_pair + (k & mask)) != 0;
+                       });
     } else {
-        // The last packet: node shares differ, place each from its own address.
-        for (uint64_t pi = 0; pi < (uint64_t{1} << r); ++pi) {
+        for (uint64_t base = 0; base < nodes; base += 64) {
+            const uint64_t pi = base + lane_id();
+            if (pi >= nodes) continue;
             const NodeDesc& nd = dsc[b0 + pi];
-            const uint64_t len = packet_share(nd.cnt, packet, span_log2, P.leaf);
-            if (len == 0) continue;
-            const uint64_t start = bit_of(nd.a[side], first_pair);
-            wave_write_run(blob, secbase + start, len, [&](uint64_t k) {
-                return row_bit(rows, nd.row, first_pair + k) != 0;
-            });
+            lane_or_row(blob, secbase + bit_of(nd.a[side], first_pair), rows, nd.row, first_pair,
+                        packet_share(nd.cnt, packet, span_log2, P.leaf));
         }
     }
 }
@@ -631,25 +690,16 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
                 const SideAddr a =
                     address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
                 const uint32_t first_pair = pk << span_log2;
-                uint64_t acc = 0;
-                uint32_t len = 0;
+                uint64_t dst = secbase + bit_of(a, first_pair);
                 for (uint32_t pi = f; pi < end; ++pi) {
                     const uint32_t share = static_cast<uint32_t>(
                         packet_share(sub_count(n0, k, pi), pk, span_log2, L));
                     const uint32_t pos0 = noff[pi] + first_pair;
-                    for (uint32_t b = 0; b < share; ++b) {
-                        const uint32_t pos = pos0 + b;
-                        acc |= ((rows[pos >> 6] >> (pos & 63)) & 1) << len;
-                        ++len;
-                    }
-                }
-                if (acc == 0) continue;
-                const uint64_t dst = secbase + bit_of(a, first_pair);
-                auto* bw = reinterpret_cast<unsigned long long*>(blob) + (dst >> 6);
-                const uint32_t sh = dst & 63;
-                atomicOr(bw, static_cast<unsigned long long>(acc << sh));
-                if (sh && (acc >> (64 - sh))) {
-                    atomicOr(bw + 1, static_cast<unsigned long long>(acc >> (64 - sh)));
+                    lane_or_bits(blob, dst, share, [&](uint64_t b) {
+                        const uint32_t pos = pos0 + static_cast<uint32_t>(b);
+                        return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
+                    });
+                    dst += share;
                 }
             }
             continue;
@@ -682,19 +732,33 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
                                    return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
                                });
             } else {
-                for (uint32_t pi = first; pi < first + set_nodes; ++pi) {
-                    const uint32_t c = sub_count(n0, k, pi);
-                    if (c <= L) continue;
-                    const uint64_t len = packet_share(c, pk, span_log2, L);
-                    if (len == 0) continue;
-                    const uint64_t path = (b0 << k) | pi;
-                    const SideAddr a = address_at(
-                        P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
-                    const uint32_t base = noff[pi] + static_cast<uint32_t>(first_pair);
-                    wave_write_run(blob, secbase + bit_of(a, first_pair), len, [&](uint64_t q) {
-                        const uint32_t pos = base + static_cast<uint32_t>(q);
-                        return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
-                    });
+                // Node shares of this packet differ (its last packet, or a
+                // level mixing leaves and internal nodes): the run starts at
+                // the set's first internal node and every node continues
+                // where the previous one ended; lanes take nodes, a wave
+                // prefix sum places them.
+                const uint32_t end = first + set_nodes;
+                uint32_t f = first;
+                while (f < end && sub_count(n0, k, f) <= L) ++f;  // wave-uniform
+                if (f == end) continue;
+                const uint64_t path = (b0 << k) | f;
+                const SideAddr a = address_at(
+                    P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
+                const uint64_t run0 = secbase + bit_of(a, first_pair);
+                uint64_t carry = 0;
+                for (uint32_t base = f; base < end; base += 64) {
+                    const uint32_t pi = base + lane_id();
+                    const uint64_t share =
+                        pi < end ? packet_share(sub_count(n0, k, pi), pk, span_log2, L) : 0;
+                    const uint64_t incl = wave_inclusive_sum(share);
+                    if (share) {
+                        const uint32_t pos0 = noff[pi] + static_cast<uint32_t>(first_pair);
+                        lane_or_bits(blob, run0 + carry + incl - share, share, [&](uint64_t b) {
+                            const uint32_t pos = pos0 + static_cast<uint32_t>(b);
+                            return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
+                        });
+                    }
+                    carry += __shfl(incl, 63);
                 }
             }
         }
