@@ -316,6 +316,37 @@ def served(count: int, rate_rps: float, seed: int = 5005, lanes: int = 3, max_ba
             "each_result_equals_plain": same_p and same_b}
 
 
+def large_u64(seed: int = 0x5EED, reps: int = 2):
+    """N = 2^32 + 12345 uint64 entries (34 GB resident in HBM), middle_group_log2
+    5, default clusters {14,14}/{14,14}, ranks embedded in the output section.
+    The input is the keyed pseudo-random bijection generated on the device (a
+    sequential Fisher-Yates draw of 4.3e9 entries would take minutes on the
+    host); its blobs are checked by tests/test_gpu_large.py."""
+    n = (1 << 32) + 12345
+    fmt = B.FormatOptions(5, (), (), B.EMBEDDED_OUTPUT)
+    size = B.encoded_size(n, fmt)
+    d_perm = torch.empty(n, dtype=torch.int64, device="cuda")
+    d_out = torch.empty((size + 7) // 8 * 8, dtype=torch.uint8, device="cuda")
+    t0 = time.perf_counter()
+    B.generate_device(B.KEYED, n, seed, d_perm.data_ptr(), 8)
+    gen_s = time.perf_counter() - t0
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        B.compress_device(d_perm.data_ptr(), n, 8, d_out.data_ptr(), d_out.numel(), fmt)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
+    del d_perm, d_out
+    torch.cuda.empty_cache()
+    B.release_workspace()
+    return {"workload": "plain compress, keyed bijection of 2^32+12345 uint64, g=5, default "
+                        "clusters, embedded_output", "n": n, "seed": seed, "blob_bytes": size,
+            "ms": dt * 1e3, "Mentries_per_s": n / dt / 1e6, "generate_s": gen_s,
+            "reps": reps, "checked_by": "tests/test_gpu_large.py (closed forms, round trips)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,6 +358,7 @@ def main():
     ap.add_argument("--batch-items", type=int, default=4096)
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--offered-rps", type=float, default=400.0)
+    ap.add_argument("--no-large", action="store_true", help="skip the 2^32+12345 u64 run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -429,6 +461,8 @@ def main():
         line["host_inclusive"] = host_inclusive(perm, want)
         line["batched"] = batched(args.batch_items)
         line["served"] = served(args.requests, args.offered_rps)
+        if not args.no_large:
+            line["u64_2e32"] = large_u64()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(perm, want)
     print(json.dumps(line), flush=True)
