@@ -45,6 +45,8 @@ EXPORTS = (
     "benes_hip_server_destroy",
     "benes_hip_generate_device",
     "benes_hip_release_workspace",
+    "benes_hip_autotune_report",
+    "benes_hip_set_subtree_shape",
     "benes_hip_encoded_size",
     "benes_hip_last_device_seconds",
     "benes_hip_set_profiling",
@@ -170,6 +172,8 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_generate.argtypes = [u32, u64, u64, u32, vp, u32]
     lib.benes_hip_generate_device.argtypes = [u32, u64, u64, vp, u32, vp]
     lib.benes_hip_release_workspace.restype = None
+    lib.benes_hip_autotune_report.argtypes = [cp, sz]
+    lib.benes_hip_set_subtree_shape.argtypes = [u32, u32]
     _lib = lib
     return lib
 
@@ -443,6 +447,22 @@ def generate_device(kind: int, n: int, seed: int, d_out_ptr: int, width: int = 8
                                              ctypes.c_void_p(stream or 0))
     if rc != OK:
         raise ValueError("invalid device generator arguments")
+
+
+def autotune_report() -> str:
+    """The subtree-shape tuner's classes, device times and choices."""
+    buf = ctypes.create_string_buffer(8192)
+    library().benes_hip_autotune_report(buf, 8192)
+    return buf.value.decode()
+
+
+SUBTREE_SHAPES = ((14, 1024), (13, 1024), (13, 512), (12, 512))
+
+
+def set_subtree_shape(log2_entries: int = 0, threads: int = 0) -> None:
+    """Pins a subtree shape (one of SUBTREE_SHAPES); (0, 0) returns to tuning."""
+    if library().benes_hip_set_subtree_shape(log2_entries, threads) != OK:
+        raise ValueError(f"no subtree shape {log2_entries}x{threads}")
 
 
 def release_workspace() -> None:

@@ -14,10 +14,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <deque>
+#include <map>
 #include <thread>
 #include <unordered_map>
 #include <cstring>
@@ -124,12 +127,113 @@ Workspace& workspace_for(int dev) {
 uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 uint64_t round8(uint64_t a) { return (a + 7) & ~uint64_t{7}; }
 
-// First depth whose largest node fits one LDS subtree.
-uint32_t fusion_depth(uint64_t n, uint32_t routing_depth) {
+// Subtree shapes the subtree kernel is built for: entries log2, threads.
+struct SubtreeShape {
+    uint32_t log2;
+    uint32_t threads;
+};
+constexpr SubtreeShape kShapes[] = {{14, 1024}, {13, 1024}, {13, 512}, {12, 512}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+// First depth whose largest node fits one LDS subtree of 2^sub_log2 entries.
+uint32_t fusion_depth(uint64_t n, uint32_t routing_depth, uint32_t sub_log2) {
     uint32_t d = 0;
-    while (ceil_div(n, uint64_t{1} << d) > kSub) ++d;
+    while (ceil_div(n, uint64_t{1} << d) > (uint64_t{1} << sub_log2)) ++d;
     return std::min(d, routing_depth);
 }
+
+// Online choice of the subtree shape.  Calls fall into classes by the log2
+// of their largest item and of their total entries; the first calls of a
+// class try each shape once (any shape gives the same bytes), later calls use
+// the one with the least device time seen.  BENES_HIP_SUBTREE=<log2>x<threads>
+// pins a shape.
+class Tuner {
+public:
+    static Tuner& instance() {
+        static Tuner* t = new Tuner();
+        return *t;
+    }
+    template <typename Jobs>
+    static uint32_t class_of(const Jobs& jobs) {
+        uint64_t total = 0, most = 0;
+        for (const auto& j : jobs) {
+            total += j.n;
+            most = std::max(most, j.n);
+        }
+        auto lg = [](uint64_t v) { return v ? 63u - static_cast<uint32_t>(__builtin_clzll(v)) : 0u; };
+        return lg(most) * 64 + lg(total);
+    }
+    // Pins shape `i`, or returns to tuning when i < 0.
+    void pin(int i) { forced_.store(i); }
+    int pick(uint32_t cls) {
+        const int forced = forced_.load();
+        if (forced >= 0) return forced;
+        std::lock_guard<std::mutex> lock(mu_);
+        Entry& e = table_[cls];
+        if (e.tried < kNumShapes) return e.tried;
+        return best(e);
+    }
+    void report(uint32_t cls, int shape, double device_ms) {
+        if (forced_.load() >= 0 || device_ms <= 0) return;
+        std::lock_guard<std::mutex> lock(mu_);
+        Entry& e = table_[cls];
+        if (e.ms[shape] == 0 || device_ms < e.ms[shape]) e.ms[shape] = device_ms;
+        if (e.tried < kNumShapes && shape == e.tried) ++e.tried;
+    }
+    std::string describe() {
+        std::lock_guard<std::mutex> lock(mu_);
+        std::string out;
+        char line[256];
+        const int forced = forced_.load();
+        if (forced >= 0) {
+            std::snprintf(line, sizeof(line), "pinned %ux%u\n", kShapes[forced].log2,
+                          kShapes[forced].threads);
+            out += line;
+        }
+        for (const auto& kv : table_) {
+            const Entry& e = kv.second;
+            std::snprintf(line, sizeof(line), "largest 2^%u total 2^%u:", kv.first / 64,
+                          kv.first % 64);
+            out += line;
+            for (int i = 0; i < kNumShapes; ++i) {
+                std::snprintf(line, sizeof(line), " %ux%u=%.3fms", kShapes[i].log2,
+                              kShapes[i].threads, e.ms[i]);
+                out += line;
+            }
+            std::snprintf(line, sizeof(line), " -> %ux%u\n",
+                          kShapes[e.tried < kNumShapes ? e.tried : best(e)].log2,
+                          kShapes[e.tried < kNumShapes ? e.tried : best(e)].threads);
+            out += line;
+        }
+        return out;
+    }
+
+private:
+    struct Entry {
+        int tried = 0;
+        double ms[kNumShapes] = {};
+    };
+    Tuner() {
+        if (const char* v = std::getenv("BENES_HIP_SUBTREE")) {
+            unsigned lg = 0, th = 0;
+            if (std::sscanf(v, "%ux%u", &lg, &th) == 2) {
+                for (int i = 0; i < kNumShapes; ++i) {
+                    if (kShapes[i].log2 == lg && kShapes[i].threads == th) forced_.store(i);
+                }
+            }
+        }
+    }
+    static int best(const Entry& e) {
+        int b = 0;
+        for (int i = 1; i < kNumShapes; ++i) {
+            if (e.ms[i] > 0 && (e.ms[b] == 0 || e.ms[i] < e.ms[b])) b = i;
+        }
+        return b;
+    }
+    std::mutex mu_;
+    std::map<uint32_t, Entry> table_;
+    std::atomic<int> forced_{-1};
+};
 
 struct Profiler {
     bool on = false;
@@ -251,7 +355,9 @@ struct SegList {
 // in ws.errs.
 // `out` (8-byte aligned, total_words * 8 bytes) receives the blobs, or null
 // for the workspace's own blob buffer.
-void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_t* out) {
+void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_t* out,
+                   int shape) {
+    const SubtreeShape sub = kShapes[shape];
     const hipStream_t st = ws.stream;
     const uint32_t count = static_cast<uint32_t>(jobs.size());
     uint64_t total_n = 0, total_words = 0;
@@ -260,7 +366,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     for (Job& j : jobs) {
         j.base = total_n;
         j.blob = total_words;
-        j.df = fusion_depth(j.n, j.lay.plan.depth);
+        j.df = fusion_depth(j.n, j.lay.plan.depth, sub.log2);
         total_n += j.n;
         total_words += ceil_div(j.lay.plan.end_offset, 8);
         max_df = std::max(max_df, j.df);
@@ -352,7 +458,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     }
     const Launch subtree = subtree_list.commit(table);
     // One dispatch holds at most 2^32 - 1 work-items.
-    uint64_t most = subtree.blocks * kSubThreads;
+    uint64_t most = subtree.blocks * sub.threads;
     for (const LevelPlan& lp : levels) {
         most = std::max({most, lp.hbm.blocks * kBlock, lp.lds.blocks * kLevelThreads,
                          lp.pack[0].blocks * kBlock, lp.pack[1].blocks * kBlock});
@@ -460,9 +566,17 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     }
     // An item with fusion depth df was last written at depth df - 1, into
     // level[(df - 1) & 1]; items with df == 0 read their own source.
-    BH_LAUNCH(prof, "k_subtree", k_subtree, grid(subtree), dim3(kSubThreads), 0, st, ditems,
-              segp(subtree), subtree.count, ws.level[0].as<uint32_t>(),
-              ws.level[1].as<uint32_t>(), blob, errs);
+    auto subtree_kernel = [&](auto kernel) {
+        BH_LAUNCH(prof, "k_subtree", kernel, grid(subtree), dim3(sub.threads), 0, st, ditems,
+                  segp(subtree), subtree.count, ws.level[0].as<uint32_t>(),
+                  ws.level[1].as<uint32_t>(), blob, errs);
+    };
+    switch (sub.log2 * 10000 + sub.threads) {
+        case 141024: subtree_kernel(k_subtree<14, 1024>); break;
+        case 131024: subtree_kernel(k_subtree<13, 1024>); break;
+        case 130512: subtree_kernel(k_subtree<13, 512>); break;
+        default: subtree_kernel(k_subtree<12, 512>); break;
+    }
     BH_CHECK(hipGetLastError());
 }
 
@@ -574,8 +688,10 @@ void run_batch(Workspace& ws, std::vector<Job>& jobs, uint8_t* out, hipStream_t 
         prof.on = g_profiling;
     }
     prof.stream = s;
+    const uint32_t cls = Tuner::class_of(jobs);
+    const int shape = Tuner::instance().pick(cls);
     BH_CHECK(hipEventRecord(ws.ev_begin, s));
-    enqueue_batch(ws, jobs, prof, out);
+    enqueue_batch(ws, jobs, prof, out, shape);
     BH_CHECK(hipEventRecord(ws.ev_end, s));
     flags.assign(jobs.size(), 0);
     BH_CHECK(hipMemcpyAsync(flags.data(), ws.errs.ptr, sizeof(int) * jobs.size(),
@@ -585,6 +701,7 @@ void run_batch(Workspace& ws, std::vector<Job>& jobs, uint8_t* out, hipStream_t 
     float ms = 0;
     BH_CHECK(hipEventElapsedTime(&ms, ws.ev_begin, ws.ev_end));
     g_device_seconds = ms / 1e3;
+    Tuner::instance().report(cls, shape, ms);
     if (device_seconds) *device_seconds = g_device_seconds;
     if (user_stream) stream_after(user_stream, s);
 }
@@ -719,6 +836,7 @@ struct Lane {
     Workspace ws;
     std::vector<Request*> batch;  // set by the scheduler, emptied by the lane thread
     std::vector<Job> jobs;
+    std::vector<int> outcome;     // per request of the batch, published under the lock
     std::thread thread;
     std::condition_variable cv;  // wakes the lane thread (waits on Server::mu)
     bool busy = false;           // a batch is assigned and not yet finished
@@ -806,27 +924,33 @@ void Server::process(Lane& lane) {
     }
     prof.stream = s;
     t0 = std::chrono::steady_clock::now();
-    enqueue_batch(lane.ws, lane.jobs, prof, nullptr);
+    const uint32_t cls = Tuner::class_of(lane.jobs);
+    const int shape = Tuner::instance().pick(cls);
+    enqueue_batch(lane.ws, lane.jobs, prof, nullptr, shape);
     note_host_time("host:serve_enqueue", t0);
     std::vector<int> flags(batch.size());
     BH_CHECK(hipMemcpyAsync(flags.data(), lane.ws.errs.ptr, sizeof(int) * batch.size(),
                             hipMemcpyDeviceToHost, s));
     BH_CHECK(hipStreamSynchronize(s));
+    Tuner::instance().report(
+        cls, shape,
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     note_host_time("host:serve_device", t0);
     prof.collect();
+    // Outcomes stay with the lane until the blobs are in the callers' buffers;
+    // a request's status is only set under the server lock afterwards, so a
+    // waiter never sees it finished while its buffer is still being written.
     std::vector<HostPiece> out;
+    lane.outcome.assign(batch.size(), kOk);
     for (size_t k = 0; k < batch.size(); ++k) {
-        Request* r = batch[k];
+        const Request* r = batch[k];
         const uint64_t end = r->job.lay.plan.end_offset;
         if (flags[k]) {
-            r->status = kInvalidArgument;
-            r->message = kNotPermutation;
+            lane.outcome[k] = kInvalidArgument;
         } else if (!r->out || r->out_capacity < end) {
-            r->status = kBufferTooSmall;
-            r->message = "output buffer is too small";
+            lane.outcome[k] = kBufferTooSmall;
         } else {
             out.push_back(HostPiece{r->out, lane.jobs[k].blob * 8, end});
-            r->status = kOk;
         }
     }
     t0 = std::chrono::steady_clock::now();
@@ -852,12 +976,16 @@ void Server::serve(Lane& lane) {
         }
         lock.lock();
         const auto now = std::chrono::steady_clock::now();
-        for (Request* r : lane.batch) {
+        for (size_t k = 0; k < lane.batch.size(); ++k) {
+            Request* r = lane.batch[k];
             if (!error.empty()) {
                 r->status = kRuntime;
                 r->message = error;
-            } else if (r->status == kOk) {
-                entries += r->job.n;
+            } else {
+                r->status = lane.outcome[k];
+                if (r->status == kOk) entries += r->job.n;
+                if (r->status == kInvalidArgument) r->message = kNotPermutation;
+                if (r->status == kBufferTooSmall) r->message = "output buffer is too small";
             }
             r->done = now;
             ++completed;
@@ -1043,6 +1171,29 @@ int benes_hip_generate_device(uint32_t kind, uint64_t n, uint64_t seed, void* d_
         return BENES_HIP_RUNTIME;
     }
     return BENES_HIP_OK;
+}
+
+int benes_hip_set_subtree_shape(uint32_t log2_entries, uint32_t threads) {
+    if (log2_entries == 0 && threads == 0) {
+        Tuner::instance().pin(-1);
+        return BENES_HIP_OK;
+    }
+    for (int i = 0; i < kNumShapes; ++i) {
+        if (kShapes[i].log2 == log2_entries && kShapes[i].threads == threads) {
+            Tuner::instance().pin(i);
+            return BENES_HIP_OK;
+        }
+    }
+    return BENES_HIP_INVALID_ARGUMENT;
+}
+
+int benes_hip_autotune_report(char* out, size_t len) {
+    const std::string text = Tuner::instance().describe();
+    if (out && len) {
+        std::strncpy(out, text.c_str(), len - 1);
+        out[len - 1] = 0;
+    }
+    return static_cast<int>(text.size());
 }
 
 void benes_hip_release_workspace(void) {

@@ -38,9 +38,10 @@
 namespace benes_hip {
 
 constexpr uint32_t kBlock = 256;          // threads per block of level kernels
-constexpr uint32_t kSubLog2 = 14;          // largest subtree handled in LDS
-constexpr uint32_t kSub = 1u << kSubLog2;
-constexpr uint32_t kSubThreads = 1024;
+// Subtree shapes (entries log2, threads): k_subtree is instantiated for each;
+// the host picks one per call (smaller subtrees use less LDS, so more
+// workgroups share a CU, at the price of one more level outside LDS).
+constexpr uint32_t kSubLog2Max = 14;       // largest subtree handled in LDS
 
 struct NodeDesc {
     uint64_t off;  // absolute offset of the node's entries in the shared arrays
@@ -562,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ ite
 // ---------------------------------------------------------------------------
 // Subtrees in LDS
 //
-// One workgroup owns a subtree of at most kSub entries (a node at the fusion
+// One workgroup owns a subtree of at most 2^SubLog2 entries (a node at the fusion
 // depth d0) and runs every remaining level in LDS: node-local permutation
 // (u16, ping-pong; while a level is coloured, the other half holds the
 // nodes' inverses), union-find words over input slots (u32), and the level's
@@ -765,7 +766,8 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
     }
 }
 
-__global__ __launch_bounds__(kSubThreads) void k_subtree(const ItemCtx* __restrict__ items,
+template <uint32_t SubLog2, uint32_t Threads>
+__global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__ items,
                                                          const Seg* __restrict__ segs,
                                                          uint32_t nsegs,
                                                          const uint32_t* __restrict__ lev0,
@@ -774,6 +776,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const ItemCtx* __restri
                                                          const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
+    constexpr uint32_t kSub = 1u << SubLog2;
     __shared__ uint16_t buf[2][kSub];
     __shared__ uint32_t uf[kSub];
     __shared__ uint64_t sbits[kSub / 64 + 2];

@@ -230,6 +230,20 @@ int benes_hip_generate(uint32_t kind, uint64_t n, uint64_t seed, uint32_t rate_p
 int benes_hip_generate_device(uint32_t kind, uint64_t n, uint64_t seed, void* d_out,
                               uint32_t width_bytes, void* stream);
 
+/* The subtree shape (entries per LDS subtree, threads per workgroup) is
+ * chosen per call by an online tuner: calls are classed by the log2 of their
+ * largest item and of their total entries, the first calls of a class try
+ * each shape once (every shape gives the same bytes), later ones use the
+ * fastest seen.  The environment variable BENES_HIP_SUBTREE=<log2>x<threads>
+ * (14x1024, 13x1024, 13x512 or 12x512) pins one.  This writes a text summary
+ * of the classes seen, the device times and the choices into `out`; returns
+ * the summary's full length.  No counterpart in the reference. */
+int benes_hip_autotune_report(char* out, size_t len);
+
+/* Pins the subtree shape for later calls (one of the four above), or hands the
+ * choice back to the tuner with (0, 0).  No counterpart in the reference. */
+int benes_hip_set_subtree_shape(uint32_t log2_entries, uint32_t threads);
+
 /* Frees the device buffers the library keeps for the calling thread's
  * current device (they grow to the largest call so far and are otherwise
  * kept for reuse).  No counterpart in the reference. */

@@ -112,6 +112,27 @@ def test_global_levels(n):
             assert B.compress(perm, S.fmt(g, pl, ic, oc)) == S.oracle_compress(perm, g, pl, ic, oc)
 
 
+@pytest.mark.parametrize("shape", B.SUBTREE_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_every_subtree_shape(shape):
+    # Every subtree shape the tuner may pick gives the same bytes: sizes
+    # around each shape's subtree and LDS-level limits, several formats.
+    B.set_subtree_shape(*shape)
+    try:
+        rng = np.random.default_rng(shape[0] * 10 + shape[1])
+        for n in (4095, 4096, 4097, 8191, 8193, 16383, 16385, 65537, 100003):
+            for g, pl in ((1, 0), (3, 1), (5, 2)):
+                d = S.routing_depth(n, g)
+                for ic, oc in (((), ()), ((2, d - 3, 1), (d - 1, 1))):
+                    perm = rng.permutation(n).astype(np.uint32)
+                    want = S.oracle_compress(perm, g, pl, ic, oc)
+                    assert B.compress(perm, S.fmt(g, pl, ic, oc)) == want, (n, g, pl, ic, oc)
+        rec = next(r for r in load_fixtures() if r["name"] == "cfg1-fy-2^20")
+        perm = B.generate(rec["kind"], rec["n"], rec["seed"], rec["rate"], rec["width"])
+        assert hashlib.sha256(B.compress(perm)).hexdigest() == rec["sha256"]
+    finally:
+        B.set_subtree_shape(0, 0)
+
+
 def test_structured_inputs_large():
     n = 300007
     for kind in (B.SINGLE_CYCLE, B.NEAR_IDENTITY, B.IDENTITY, B.REVERSE):
