@@ -139,13 +139,20 @@ __device__ __forceinline__ void uf_link(W* uf, W ru, uint32_t pu, W rv, uint32_t
     }
 }
 
-// Union of two input slots.
+// Union of two input slots.  The larger slot is first hooked straight under
+// the smaller one: while it is still a root that one CAS settles the edge
+// (a root may hang under any smaller slot; the merged tree keeps its
+// smallest slot as root), and neither endpoint's tree is walked.  Only when
+// it already has a parent are both roots found and linked.
 template <typename F, typename W>
 __device__ __forceinline__ void uf_union_slots(W* uf, W a, W b, uint32_t label) {
-    uint32_t pa, pb;
-    const W ra = uf_find<F>(uf, a, pa);
-    const W rb = uf_find<F>(uf, b, pb);
-    uf_link<F>(uf, ra, pa, rb, pb, label);
+    const W hi = a > b ? a : b, lo = a > b ? b : a;
+    const W found = F::hook(uf + hi, (lo << 1) | static_cast<W>(label & 1));
+    if (found == UfWord<W>::root) return;
+    uint32_t pl, ph;
+    const W rl = uf_find<F>(uf, lo, pl);
+    const W rh = uf_find<F>(uf, found >> 1, ph);
+    uf_link<F>(uf, rl, pl, rh, ph ^ static_cast<uint32_t>(found & 1), label);
 }
 
 // Slot numbering inside a node of `count` entries (h = count/2 pairs).
