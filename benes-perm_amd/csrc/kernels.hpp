@@ -860,17 +860,46 @@ __global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__
         // Union of the two input slots reaching each output pair (the same
         // input-slot forest as the HBM levels).  Items are (node, 64-pair
         // chunk).
+        // Each lane takes kUnionBatch pairs at a time and issues their first
+        // hooks together (most settle there), so the LDS round trips of
+        // independent unions overlap.
         const uint32_t ppn = ((cmax >> 1) + 1 + 63) >> 6;
-        for (uint32_t item = wave; item < nodes * ppn; item += nwaves) {
-            const uint32_t pi = item / ppn, chunk = item - pi * ppn;
-            const uint32_t c = sub_count(n0, k, pi);
-            if (c <= L) continue;
-            const uint32_t q = chunk * 64 + lane;
-            if (q >= (c >> 1)) continue;
-            const uint32_t off = noff[pi];
-            const uint32_t a = next[off + 2 * q], b = next[off + 2 * q + 1];
-            const uint32_t sa = input_slot<uint32_t>(a, c), sb = input_slot<uint32_t>(b, c);
-            if (sa != sb) uf_union_slots<GroupForest>(uf + off, sa, sb, 1u ^ ((a ^ b) & 1u));
+        constexpr uint32_t kUnionBatch = 4;
+        for (uint32_t item0 = wave; item0 < nodes * ppn; item0 += kUnionBatch * nwaves) {
+            uint32_t base[kUnionBatch], hi[kUnionBatch], lo[kUnionBatch], found[kUnionBatch];
+            bool act[kUnionBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnionBatch; ++u) {
+                const uint32_t item = item0 + u * nwaves;
+                act[u] = false;
+                if (item >= nodes * ppn) continue;
+                const uint32_t pi = item / ppn, chunk = item - pi * ppn;
+                const uint32_t c = sub_count(n0, k, pi);
+                const uint32_t q = chunk * 64 + lane;
+                if (c <= L || q >= (c >> 1)) continue;
+                const uint32_t off = noff[pi];
+                const uint32_t a = next[off + 2 * q], b = next[off + 2 * q + 1];
+                const uint32_t sa = input_slot<uint32_t>(a, c), sb = input_slot<uint32_t>(b, c);
+                if (sa == sb) continue;
+                act[u] = true;
+                base[u] = off;
+                hi[u] = sa > sb ? sa : sb;
+                lo[u] = ((sa > sb ? sb : sa) << 1) | (1u ^ ((a ^ b) & 1u));
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnionBatch; ++u) {
+                if (act[u]) found[u] = GroupForest::hook(uf + base[u] + hi[u], lo[u]);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnionBatch; ++u) {
+                if (!act[u] || found[u] == UfWord<uint32_t>::root) continue;
+                // hi already had a parent: join the two trees by their roots.
+                uint32_t pl, ph;
+                uint32_t* const f = uf + base[u];
+                const uint32_t rl = uf_find<GroupForest>(f, lo[u] >> 1, pl);
+                const uint32_t rh = uf_find<GroupForest>(f, found[u] >> 1, ph);
+                uf_link<GroupForest>(f, rl, pl, rh, ph ^ (found[u] & 1u), lo[u] & 1u);
+            }
         }
         __syncthreads();
         // Input switch values: each pair's parity to its root.
