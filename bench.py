@@ -139,6 +139,24 @@ def other_inputs(d_out, size, stream, steps: int):
     return out
 
 
+def device_lookup(d_out, size: int, d_perm, reps: int = 3):
+    """Whole decode of the 2^26 blob on the device (the reference's
+    forward_lookup for every entry), checked against the input."""
+    d_dec = torch.empty(N, dtype=torch.int32, device="cuda")
+    B.lookup_device(d_out.data_ptr(), size, None, N, d_dec.data_ptr(), 4)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        B.lookup_device(d_out.data_ptr(), size, None, N, d_dec.data_ptr(), 4)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
+    ok = bool(torch.equal(d_dec, d_perm))
+    del d_dec
+    return {"lookups": N, "ms": dt * 1e3, "Mlookups_per_s": N / dt / 1e6, "round_trip_exact": ok}
+
+
 def host_inclusive(perm: np.ndarray, want: str | None, reps: int = 3):
     """benes_hip_compress from a host span to a host blob (PCIe both ways):
     the reference's own call shape, reported beside the device-resident rate."""
@@ -461,6 +479,7 @@ def main():
         "setup": {"generate_s": gen_s},
     }
     if not args.no_extra:
+        line["lookup_2e26"] = device_lookup(d_out, size, d_perm)
         line["inputs_2e26"] = other_inputs(d_out, size, stream, max(3, args.steps // 2))
         line["host_inclusive"] = host_inclusive(perm, want)
         line["batched"] = batched(args.batch_items)

@@ -25,7 +25,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbenes_hip.so")
 
-OK, INVALID_ARGUMENT, OVERFLOW, LENGTH, LOGIC, RUNTIME, BUFFER_TOO_SMALL, PENDING = range(8)
+(OK, INVALID_ARGUMENT, OVERFLOW, LENGTH, LOGIC, RUNTIME, BUFFER_TOO_SMALL, PENDING,
+ OUT_OF_RANGE) = range(9)
 
 SEPARATE, EMBEDDED_INPUT, EMBEDDED_OUTPUT = 0, 1, 2
 FISHER_YATES, SINGLE_CYCLE, NEAR_IDENTITY, IDENTITY, REVERSE, KEYED = range(6)
@@ -47,6 +48,8 @@ EXPORTS = (
     "benes_hip_release_workspace",
     "benes_hip_autotune_report",
     "benes_hip_set_subtree_shape",
+    "benes_hip_lookup",
+    "benes_hip_lookup_device",
     "benes_hip_encoded_size",
     "benes_hip_last_device_seconds",
     "benes_hip_set_profiling",
@@ -174,6 +177,8 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_release_workspace.restype = None
     lib.benes_hip_autotune_report.argtypes = [cp, sz]
     lib.benes_hip_set_subtree_shape.argtypes = [u32, u32]
+    lib.benes_hip_lookup.argtypes = [vp, u64, vp, u64, ctypes.c_int, vp, u32, cp, sz]
+    lib.benes_hip_lookup_device.argtypes = [vp, u64, vp, u64, ctypes.c_int, vp, u32, vp, cp, sz]
     _lib = lib
     return lib
 
@@ -185,6 +190,7 @@ _ERRORS = {
     LOGIC: AssertionError,
     RUNTIME: RuntimeError,
     BUFFER_TOO_SMALL: BufferError,
+    OUT_OF_RANGE: IndexError,
 }
 
 
@@ -384,6 +390,30 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def lookup(blob, values, inverse: bool = False) -> np.ndarray:
+    """Images of `values` (host) under the permutation a host blob encodes
+    (or under its inverse), decoded on the GPU."""
+    buf = np.frombuffer(blob, np.uint8) if isinstance(blob, (bytes, bytearray)) else \
+        np.ascontiguousarray(blob, np.uint8)
+    vals = np.ascontiguousarray(values, np.uint64)
+    out = np.empty(vals.size, np.uint64)
+    err = ctypes.create_string_buffer(256)
+    _raise(library().benes_hip_lookup(buf.ctypes.data, buf.size, vals.ctypes.data, vals.size,
+                                      1 if inverse else 0, out.ctypes.data, 8, err, 256), err)
+    return out
+
+
+def lookup_device(d_blob_ptr: int, blob_size: int, d_values_ptr: int | None, count: int,
+                  d_out_ptr: int, width: int = 8, inverse: bool = False,
+                  stream: int | None = None) -> None:
+    """Device lookups; d_values_ptr None means values 0..count-1."""
+    err = ctypes.create_string_buffer(256)
+    _raise(library().benes_hip_lookup_device(ctypes.c_void_p(d_blob_ptr), blob_size,
+                                             ctypes.c_void_p(d_values_ptr or 0), count,
+                                             1 if inverse else 0, ctypes.c_void_p(d_out_ptr),
+                                             width, ctypes.c_void_p(stream or 0), err, 256), err)
 
 
 def last_device_seconds() -> float:

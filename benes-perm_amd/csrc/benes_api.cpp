@@ -18,6 +18,7 @@ namespace {
     case BENES_HIP_OVERFLOW: throw std::overflow_error(message);
     case BENES_HIP_LENGTH: throw std::length_error(message);
     case BENES_HIP_LOGIC: throw std::logic_error(message);
+    case BENES_HIP_OUT_OF_RANGE: throw std::out_of_range(message);
     default: throw std::runtime_error(message);
     }
 }

@@ -729,6 +729,46 @@ int with_workspace(Status& st, Body&& body) {
     return st.code;
 }
 
+// Lookups of `count` values (d_values null: 0..count-1) in a device blob, on
+// the workspace's stream ordered after `user_stream`.
+void run_lookup(Workspace& ws, const uint8_t* d_blob, uint64_t blob_size,
+                const uint64_t* d_values, uint64_t count, int inverse, void* d_out,
+                uint32_t width, hipStream_t user_stream, Status& st) {
+    const hipStream_t s = ws.stream;
+    if (user_stream) stream_after(s, user_stream);
+    uint8_t hdr[20 + 64];
+    const uint64_t avail = std::min<uint64_t>(blob_size, sizeof(hdr));
+    BH_CHECK(hipMemcpyAsync(hdr, d_blob, avail, hipMemcpyDeviceToHost, s));
+    BH_CHECK(hipStreamSynchronize(s));
+    Layout lay;
+    if (!parse_header(hdr, avail, blob_size, lay, st)) return;
+    if (width == 4 && lay.plan.n > 0x100000000ull) {
+        st.set(kInvalidArgument, "lookup results need 64-bit entries for this size");
+        return;
+    }
+    if (count == 0) return;
+    ws.plans.ensure(sizeof(Plan));
+    ws.errs.ensure(sizeof(int));
+    BH_CHECK(hipMemcpyAsync(ws.plans.ptr, &lay.plan, sizeof(Plan), hipMemcpyHostToDevice, s));
+    BH_CHECK(hipMemsetAsync(ws.errs.ptr, 0, sizeof(int), s));
+    const uint64_t blocks = std::min<uint64_t>(ceil_div(count, kBlock), uint64_t{1} << 20);
+    if (width == 4) {
+        hipLaunchKernelGGL(k_lookup<uint32_t>, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock),
+                           0, s, ws.plans.as<Plan>(), d_blob, d_values, count, inverse,
+                           static_cast<uint32_t*>(d_out), ws.errs.as<int>());
+    } else {
+        hipLaunchKernelGGL(k_lookup<uint64_t>, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock),
+                           0, s, ws.plans.as<Plan>(), d_blob, d_values, count, inverse,
+                           static_cast<uint64_t*>(d_out), ws.errs.as<int>());
+    }
+    BH_CHECK(hipGetLastError());
+    int flag = 0;
+    BH_CHECK(hipMemcpyAsync(&flag, ws.errs.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    BH_CHECK(hipStreamSynchronize(s));
+    if (flag) st.set(kOutOfRange, "permutation lookup is out of range");
+    if (user_stream) stream_after(user_stream, s);
+}
+
 int compress_one(const void* perm, bool on_device, uint64_t n, uint32_t width,
                  const benes_hip_format* format, uint8_t* dst_host, uint8_t* dst_dev,
                  uint64_t capacity, uint64_t* out_size, hipStream_t user_stream,
@@ -1171,6 +1211,52 @@ int benes_hip_generate_device(uint32_t kind, uint64_t n, uint64_t seed, void* d_
         return BENES_HIP_RUNTIME;
     }
     return BENES_HIP_OK;
+}
+
+int benes_hip_lookup_device(const uint8_t* d_blob, uint64_t blob_size, const uint64_t* d_values,
+                            uint64_t count, int inverse, void* d_out, uint32_t width_bytes,
+                            void* stream, char* err, size_t err_len) {
+    Status st;
+    if ((width_bytes != 4 && width_bytes != 8) || !d_blob || (count && !d_out)) {
+        st.set(kInvalidArgument, "invalid lookup arguments");
+        return finish(st, err, err_len);
+    }
+    with_workspace(st, [&](Workspace& ws) {
+        run_lookup(ws, d_blob, blob_size, d_values, count, inverse, d_out, width_bytes,
+                   static_cast<hipStream_t>(stream), st);
+    });
+    return finish(st, err, err_len);
+}
+
+int benes_hip_lookup(const uint8_t* blob, uint64_t blob_size, const uint64_t* values,
+                     uint64_t count, int inverse, void* out, uint32_t width_bytes, char* err,
+                     size_t err_len) {
+    Status st;
+    if ((width_bytes != 4 && width_bytes != 8) || !blob || (count && (!out || !values))) {
+        st.set(kInvalidArgument, "invalid lookup arguments");
+        return finish(st, err, err_len);
+    }
+    with_workspace(st, [&](Workspace& ws) {
+        ws.blob.ensure(std::max<uint64_t>(blob_size, 1));
+        ws.input.ensure(std::max<uint64_t>(count * 8, 8));
+        ws.inv0.ensure(std::max<uint64_t>(count * width_bytes, 8));
+        ws.ring.upload(ws.blob.as<uint8_t>(),
+                       {HostPiece{const_cast<uint8_t*>(blob), 0, blob_size}}, ws.stream);
+        if (count) {
+            ws.ring.upload(ws.input.as<uint8_t>(),
+                           {HostPiece{reinterpret_cast<uint8_t*>(const_cast<uint64_t*>(values)),
+                                      0, count * 8}},
+                           ws.stream);
+        }
+        run_lookup(ws, ws.blob.as<uint8_t>(), blob_size, ws.input.as<uint64_t>(), count,
+                   inverse, ws.inv0.ptr, width_bytes, nullptr, st);
+        if (st.ok() && count) {
+            ws.ring.download(ws.inv0.as<uint8_t>(),
+                             {HostPiece{static_cast<uint8_t*>(out), 0, count * width_bytes}},
+                             ws.stream);
+        }
+    });
+    return finish(st, err, err_len);
 }
 
 int benes_hip_set_subtree_shape(uint32_t log2_entries, uint32_t threads) {

@@ -561,6 +561,146 @@ __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ ite
 }
 
 // ---------------------------------------------------------------------------
+// Lookup (the reference's lookup, internal.hpp:1036-1124): one thread per
+// value.  Forward: descend through the input switches (the odd tail always
+// goes left), decode the leaf's factoradic rank, climb back through the output
+// switches; inverse swaps the two sides.  Addresses come from the same closed
+// forms the encoder uses; each climb step recomputes its node's address, so a
+// lookup costs O(depth^2) address steps and no per-thread stack.
+
+__device__ __forceinline__ uint32_t blob_bit(const uint8_t* blob, uint64_t bit) {
+    return (blob[bit >> 3] >> (bit & 7)) & 1u;
+}
+
+// Eight 16-bit limbs (least significant first) of the `bits` (<= 118) bits
+// of the blob starting at bit `at`.
+__device__ __forceinline__ void read_rank_limbs(const uint8_t* blob, uint64_t at, uint32_t bits,
+                                                uint32_t limb[8]) {
+    uint64_t lo = 0, hi = 0;
+    if (bits == 0) {
+#pragma unroll
+        for (uint32_t l = 0; l < 8; ++l) limb[l] = 0;
+        return;
+    }
+    const uint64_t byte0 = at >> 3;
+    const uint32_t sh = at & 7;
+    const uint32_t nbytes = (sh + bits + 7) >> 3;  // at most 16
+    for (uint32_t b = 0; b < nbytes; ++b) {
+        const uint64_t v = blob[byte0 + b];
+        if (b < 8) lo |= v << (8 * b);
+        else hi |= v << (8 * (b - 8));
+    }
+    // Shift the 128-bit (hi:lo) right by sh and keep `bits` bits.
+    if (sh) {
+        lo = (lo >> sh) | (hi << (64 - sh));
+        hi >>= sh;
+    }
+    if (bits < 64) {
+        lo &= bits ? (~uint64_t{0} >> (64 - bits)) : 0;
+        hi = 0;
+    } else if (bits < 128) {
+        hi &= bits > 64 ? (~uint64_t{0} >> (128 - bits)) : 0;
+    }
+#pragma unroll
+    for (uint32_t l = 0; l < 4; ++l) {
+        limb[l] = static_cast<uint32_t>(lo >> (16 * l)) & 0xFFFFu;
+        limb[l + 4] = static_cast<uint32_t>(hi >> (16 * l)) & 0xFFFFu;
+    }
+}
+
+// Divides the limbs by k (<= 32) in place and returns the remainder.
+__device__ __forceinline__ uint32_t divide_limbs(uint32_t limb[8], uint32_t k) {
+    uint32_t rem = 0;
+#pragma unroll
+    for (int l = 7; l >= 0; --l) {
+        const uint32_t cur = (rem << 16) | limb[l];
+        limb[l] = cur / k;
+        rem = cur - limb[l] * k;
+    }
+    return rem;
+}
+
+// k-th (0-based) set bit of m.
+__device__ __forceinline__ uint32_t select_bit(uint32_t m, uint32_t k) {
+    for (uint32_t i = 0; i < k; ++i) m &= m - 1;
+    return static_cast<uint32_t>(__builtin_ctz(m));
+}
+
+template <typename VO>
+__global__ __launch_bounds__(kBlock) void k_lookup(const Plan* __restrict__ plan,
+                                                   const uint8_t* __restrict__ blob,
+                                                   const uint64_t* __restrict__ values,
+                                                   uint64_t count, int inverse,
+                                                   VO* __restrict__ out, int* __restrict__ err) {
+    const Plan& P = *plan;
+    const int down = inverse ? 1 : 0, up = 1 - down;
+    const uint64_t down_sec = (down ? P.output_offset : P.input_offset) * 8;
+    const uint64_t up_sec = (up ? P.output_offset : P.input_offset) * 8;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < count;
+         t += stride) {
+        uint64_t index = values ? values[t] : t;
+        if (index >= P.n) {
+            err[0] = 1;
+            out[t] = 0;
+            continue;
+        }
+        // Descend.
+        Walk w = walk_root(P, down);
+        uint64_t cnt = P.n, path = 0, rmask = 0;
+        uint32_t d = 0;
+        while (cnt > P.leaf) {
+            bool right = false;
+            if (!((cnt & 1) && index == cnt - 1)) {
+                const SideAddr a = address_at(P, w, d, path);
+                right = blob_bit(blob, down_sec + bit_of(a, index >> 1)) != (index & 1);
+            }
+            rmask |= static_cast<uint64_t>(right) << d;
+            const uint64_t left_cnt = cnt - (cnt >> 1);
+            path = path * 2 + (right ? 1 : 0);
+            ++d;
+            w = walk_from(P, down, w, d - 1, d, path);
+            index >>= 1;
+            cnt = right ? (cnt >> 1) : left_cnt;
+        }
+        // The leaf: Lehmer digits from the top (d_i = rank mod (i+1), then
+        // divide), entry i the d_i-th smallest value not taken by later ones.
+        const uint32_t c = static_cast<uint32_t>(cnt);
+        uint32_t limb[8];
+        read_rank_limbs(blob, leaf_section_offset(P) * 8 + leaf_rank_bit(P, d, path),
+                        rank_bits(c), limb);
+        uint32_t avail = c >= 32 ? ~0u : ((1u << c) - 1u);
+        uint64_t found = ~uint64_t{0};
+        for (uint32_t i = c; i-- > 0;) {
+            const uint32_t value = select_bit(avail, divide_limbs(limb, i + 1));
+            avail &= ~(1u << value);
+            if (!inverse && i == index) {
+                found = value;
+                break;
+            }
+            if (inverse && value == index) {
+                found = i;
+                break;
+            }
+        }
+        index = found;
+        // Climb.
+        for (uint32_t k = d; k-- > 0;) {
+            const uint64_t pk = path >> (d - k);
+            const bool right = (rmask >> k) & 1;
+            const uint64_t size = node_count(P.n, k, pk), pairs = size >> 1;
+            if (!right && (size & 1) && index == pairs) {
+                index = size - 1;
+                continue;
+            }
+            const SideAddr a = node_address(P, up, k, pk);
+            index = index * 2 + ((right ? 1u : 0u) ^ blob_bit(blob, up_sec + bit_of(a, index)));
+        }
+        out[t] = static_cast<VO>(index);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Subtrees in LDS
 //
 // One workgroup owns a subtree of at most 2^SubLog2 entries (a node at the fusion

@@ -224,6 +224,49 @@ inline bool build_layout(uint64_t n, const FormatArgs& fmt, Layout& out, Status&
 
 // Writes the blob header (internal.hpp:431-454) into `dst`, which must hold
 // layout.header_bytes bytes.
+// Reads a blob's header (`avail` bytes of it at hdr) and rebuilds its layout,
+// with the reference's checks and messages (benes.cpp:26-77); `blob_size` is
+// the whole blob's size, which must equal the layout's end.
+inline bool parse_header(const uint8_t* hdr, uint64_t avail, uint64_t blob_size, Layout& lay,
+                         Status& st) {
+    static const uint8_t magic[8] = {'B', 'E', 'N', 'E', 'S', '0', '1', 0};
+    if (avail < 20) {
+        st.set(kInvalidArgument, "compressed permutation header is truncated");
+        return false;
+    }
+    if (std::memcmp(hdr, magic, 8) != 0) {
+        st.set(kInvalidArgument, "compressed permutation has invalid magic");
+        return false;
+    }
+    uint64_t n = 0;
+    for (int i = 0; i < 8; ++i) n |= static_cast<uint64_t>(hdr[8 + i]) << (8 * i);
+    const uint32_t placement = hdr[16], g = hdr[17], ni = hdr[18], no = hdr[19];
+    const uint64_t header_bytes = 20 + ni + no;
+    if (header_bytes > 20 + 64 || header_bytes > blob_size || avail < header_bytes) {
+        st.set(kInvalidArgument, "unsupported compressed permutation header size");
+        return false;
+    }
+    if (placement > 2) {
+        st.set(kInvalidArgument, "compressed permutation has invalid format parameters");
+        return false;
+    }
+    if (n == 0) {
+        st.set(kInvalidArgument, "compressed permutation has invalid dimensions");
+        return false;
+    }
+    FormatArgs fa;
+    fa.middle_group_log2 = g;
+    fa.placement = placement;
+    fa.input_clusters.assign(hdr + 20, hdr + 20 + ni);
+    fa.output_clusters.assign(hdr + 20 + ni, hdr + 20 + ni + no);
+    if (!build_layout(n, fa, lay, st)) return false;
+    if (lay.plan.end_offset != blob_size) {
+        st.set(kInvalidArgument, "compressed permutation has invalid section bounds");
+        return false;
+    }
+    return true;
+}
+
 inline void write_header(const Layout& lay, uint8_t* dst) {
     static const uint8_t magic[8] = {'B', 'E', 'N', 'E', 'S', '0', '1', 0};
     std::memcpy(dst, magic, 8);

@@ -20,6 +20,7 @@ enum Code : int {
     kRuntime = BENES_HIP_RUNTIME,
     kBufferTooSmall = BENES_HIP_BUFFER_TOO_SMALL,
     kPending = BENES_HIP_PENDING,
+    kOutOfRange = BENES_HIP_OUT_OF_RANGE,
 };
 
 struct Status {

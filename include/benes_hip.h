@@ -39,6 +39,7 @@ extern "C" {
 #define BENES_HIP_RUNTIME 5          /* std::runtime_error (device error, unavailable) */
 #define BENES_HIP_BUFFER_TOO_SMALL 6 /* out_capacity < needed; *out_size holds the need */
 #define BENES_HIP_PENDING 7          /* benes_hip_server_wait timed out; request still running */
+#define BENES_HIP_OUT_OF_RANGE 8     /* std::out_of_range (lookup of a value >= n) */
 
 /* FormatOptions (reference include/benes/benes.hpp:32-39).  Empty cluster
  * lists (count 0) select the balanced default, as in the reference. */
@@ -147,6 +148,24 @@ int benes_hip_server_wait(benes_hip_server* server, uint64_t id, int64_t timeout
 int benes_hip_server_stats(benes_hip_server* server, benes_hip_server_counters* out);
 /* Finishes queued and running requests, then stops the loop. */
 void benes_hip_server_destroy(benes_hip_server* server);
+
+/* ---- Lookup: the consumer side of the format. ----
+ * Replaces the reference's forward_lookup / inverse_lookup
+ * (src/internal.hpp:1036-1139), as used by CompressedPermutationView
+ * (src/benes.cpp:581-700): images of `count` values under the permutation a
+ * blob encodes (inverse != 0: under its inverse).  The header is checked as
+ * the reference's parse_header does (src/benes.cpp:26-77); section padding is
+ * not checked.  A value >= n gives BENES_HIP_OUT_OF_RANGE.  Results are
+ * written as width_bytes (4 or 8) entries.
+ * _device: blob, values and results are device memory, work on `stream`;
+ * d_values == NULL looks up 0..count-1 (a whole decode when count == n).
+ * Host form: host buffers, staged through the library's pinned ring. */
+int benes_hip_lookup_device(const uint8_t* d_blob, uint64_t blob_size, const uint64_t* d_values,
+                            uint64_t count, int inverse, void* d_out, uint32_t width_bytes,
+                            void* stream, char* err, size_t err_len);
+int benes_hip_lookup(const uint8_t* blob, uint64_t blob_size, const uint64_t* values,
+                     uint64_t count, int inverse, void* out, uint32_t width_bytes, char* err,
+                     size_t err_len);
 
 /* Size of the blob for n entries (reference make_layout, src/internal.hpp:389-429). */
 int benes_hip_encoded_size(uint64_t n, const benes_hip_format* format,

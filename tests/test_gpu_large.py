@@ -17,7 +17,8 @@ do not depend on size:
     input section is all zero, the output section's ones are exactly those of
     the ranks, and sampled entries decode to themselves;
   * a keyed pseudo-random bijection: sampled entries decode, forward and
-    backward, to the values that were compressed.
+    backward, to the values that were compressed (the oracle's decoder), and
+    the device lookup decodes all 4.3e9 entries back to the input.
 Inputs are generated on the device; the permutation (34 GB) never crosses to
 the host, the blob (16.5 GB) does once per case for the oracle's decoder.
 """
@@ -137,3 +138,12 @@ def test_keyed_round_trip(big):
     want = d_perm[torch.from_numpy(idx.astype(np.int64)).cuda()].cpu().numpy().view(np.uint64)
     assert np.array_equal(S.oracle_lookup(host, idx), want)
     assert np.array_equal(S.oracle_lookup(host, want, inverse=True), idx)
+    # And every entry: the device lookup (checked against the oracle's decoder
+    # in test_gpu_lookup.py) decodes the whole blob back to the input.
+    del host
+    B.release_workspace()
+    d_dec = torch.empty(N, dtype=torch.int64, device="cuda")
+    B.lookup_device(blob.data_ptr(), blob.numel(), None, N, d_dec.data_ptr(), 8)
+    torch.cuda.synchronize()
+    assert torch.equal(d_dec, d_perm)
+    del d_dec
