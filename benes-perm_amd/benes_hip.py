@@ -480,13 +480,18 @@ def generate_device(kind: int, n: int, seed: int, d_out_ptr: int, width: int = 8
 
 
 def autotune_report() -> str:
-    """The subtree-shape tuner's classes, device times and choices."""
-    buf = ctypes.create_string_buffer(8192)
-    library().benes_hip_autotune_report(buf, 8192)
+    """The subtree-shape tuner's classes, trial device times and choices."""
+    lib = library()
+    need = lib.benes_hip_autotune_report(None, 0)
+    buf = ctypes.create_string_buffer(need + 1)
+    lib.benes_hip_autotune_report(buf, need + 1)
     return buf.value.decode()
 
 
 SUBTREE_SHAPES = ((14, 1024), (13, 1024), (13, 512), (12, 512))
+# Calls of one size class that the tuner spends on trials (two per shape);
+# BENES_HIP_AUTOTUNE_TRIAL_CALLS in benes_hip.h.
+AUTOTUNE_TRIAL_CALLS = 8
 
 
 def set_subtree_shape(log2_entries: int = 0, threads: int = 0) -> None:

@@ -143,10 +143,16 @@ uint32_t fusion_depth(uint64_t n, uint32_t routing_depth, uint32_t sub_log2) {
 }
 
 // Online choice of the subtree shape.  Calls fall into classes by the log2
-// of their largest item and of their total entries; the first calls of a
-// class try each shape once (any shape gives the same bytes), later calls use
-// the one with the least device time seen.  BENES_HIP_SUBTREE=<log2>x<threads>
-// pins a shape.
+// of their largest item and of their total entries.  The first
+// kTrialRounds * kNumShapes calls of a class are trials that cycle through the
+// shapes (any shape gives the same bytes); once they have all reported, the
+// class keeps the shape with the least device time over its trials for good,
+// so later calls of other inputs in the same class cannot move it.
+// BENES_HIP_SUBTREE=<log2>x<threads> pins a shape.
+constexpr int kTrialRounds = BENES_HIP_AUTOTUNE_TRIAL_CALLS / kNumShapes;
+static_assert(kTrialRounds * kNumShapes == BENES_HIP_AUTOTUNE_TRIAL_CALLS,
+              "trial calls are whole rounds over the shapes");
+
 class Tuner {
 public:
     static Tuner& instance() {
@@ -165,20 +171,26 @@ public:
     }
     // Pins shape `i`, or returns to tuning when i < 0.
     void pin(int i) { forced_.store(i); }
-    int pick(uint32_t cls) {
+    // The shape for the next call of class `cls`, and whether it is a trial.
+    int pick(uint32_t cls, bool& trial) {
+        trial = false;
         const int forced = forced_.load();
         if (forced >= 0) return forced;
         std::lock_guard<std::mutex> lock(mu_);
         Entry& e = table_[cls];
-        if (e.tried < kNumShapes) return e.tried;
+        if (e.issued < BENES_HIP_AUTOTUNE_TRIAL_CALLS) {
+            trial = true;
+            return e.issued++ % kNumShapes;
+        }
         return best(e);
     }
-    void report(uint32_t cls, int shape, double device_ms) {
-        if (forced_.load() >= 0 || device_ms <= 0) return;
+    // Device time of a trial call; other calls are not recorded.
+    void report(uint32_t cls, int shape, bool trial, double device_ms) {
+        if (!trial || device_ms <= 0) return;
         std::lock_guard<std::mutex> lock(mu_);
         Entry& e = table_[cls];
         if (e.ms[shape] == 0 || device_ms < e.ms[shape]) e.ms[shape] = device_ms;
-        if (e.tried < kNumShapes && shape == e.tried) ++e.tried;
+        ++e.reported;
     }
     std::string describe() {
         std::lock_guard<std::mutex> lock(mu_);
@@ -200,9 +212,14 @@ public:
                               kShapes[i].threads, e.ms[i]);
                 out += line;
             }
-            std::snprintf(line, sizeof(line), " -> %ux%u\n",
-                          kShapes[e.tried < kNumShapes ? e.tried : best(e)].log2,
-                          kShapes[e.tried < kNumShapes ? e.tried : best(e)].threads);
+            const int b = best(e);
+            if (e.reported < BENES_HIP_AUTOTUNE_TRIAL_CALLS) {
+                std::snprintf(line, sizeof(line), " (trials %d of %d)\n", e.reported,
+                              BENES_HIP_AUTOTUNE_TRIAL_CALLS);
+            } else {
+                std::snprintf(line, sizeof(line), " -> %ux%u\n", kShapes[b].log2,
+                              kShapes[b].threads);
+            }
             out += line;
         }
         return out;
@@ -210,7 +227,8 @@ public:
 
 private:
     struct Entry {
-        int tried = 0;
+        int issued = 0;
+        int reported = 0;
         double ms[kNumShapes] = {};
     };
     Tuner() {
@@ -689,7 +707,8 @@ void run_batch(Workspace& ws, std::vector<Job>& jobs, uint8_t* out, hipStream_t 
     }
     prof.stream = s;
     const uint32_t cls = Tuner::class_of(jobs);
-    const int shape = Tuner::instance().pick(cls);
+    bool trial = false;
+    const int shape = Tuner::instance().pick(cls, trial);
     BH_CHECK(hipEventRecord(ws.ev_begin, s));
     enqueue_batch(ws, jobs, prof, out, shape);
     BH_CHECK(hipEventRecord(ws.ev_end, s));
@@ -701,7 +720,7 @@ void run_batch(Workspace& ws, std::vector<Job>& jobs, uint8_t* out, hipStream_t 
     float ms = 0;
     BH_CHECK(hipEventElapsedTime(&ms, ws.ev_begin, ws.ev_end));
     g_device_seconds = ms / 1e3;
-    Tuner::instance().report(cls, shape, ms);
+    Tuner::instance().report(cls, shape, trial, ms);
     if (device_seconds) *device_seconds = g_device_seconds;
     if (user_stream) stream_after(user_stream, s);
 }
@@ -965,16 +984,19 @@ void Server::process(Lane& lane) {
     prof.stream = s;
     t0 = std::chrono::steady_clock::now();
     const uint32_t cls = Tuner::class_of(lane.jobs);
-    const int shape = Tuner::instance().pick(cls);
+    bool trial = false;
+    const int shape = Tuner::instance().pick(cls, trial);
+    BH_CHECK(hipEventRecord(lane.ws.ev_begin, s));
     enqueue_batch(lane.ws, lane.jobs, prof, nullptr, shape);
+    BH_CHECK(hipEventRecord(lane.ws.ev_end, s));
     note_host_time("host:serve_enqueue", t0);
     std::vector<int> flags(batch.size());
     BH_CHECK(hipMemcpyAsync(flags.data(), lane.ws.errs.ptr, sizeof(int) * batch.size(),
                             hipMemcpyDeviceToHost, s));
     BH_CHECK(hipStreamSynchronize(s));
-    Tuner::instance().report(
-        cls, shape,
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    float batch_ms = 0;
+    BH_CHECK(hipEventElapsedTime(&batch_ms, lane.ws.ev_begin, lane.ws.ev_end));
+    Tuner::instance().report(cls, shape, trial, batch_ms);
     note_host_time("host:serve_device", t0);
     prof.collect();
     // Outcomes stay with the lane until the blobs are in the callers' buffers;

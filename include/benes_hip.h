@@ -251,12 +251,15 @@ int benes_hip_generate_device(uint32_t kind, uint64_t n, uint64_t seed, void* d_
 
 /* The subtree shape (entries per LDS subtree, threads per workgroup) is
  * chosen per call by an online tuner: calls are classed by the log2 of their
- * largest item and of their total entries, the first calls of a class try
- * each shape once (every shape gives the same bytes), later ones use the
- * fastest seen.  The environment variable BENES_HIP_SUBTREE=<log2>x<threads>
+ * largest item and of their total entries; the first
+ * BENES_HIP_AUTOTUNE_TRIAL_CALLS calls of a class are trials that cycle twice
+ * through the shapes (every shape gives the same bytes), and later calls of
+ * the class use the shape whose fastest trial was fastest, without further
+ * updates.  The environment variable BENES_HIP_SUBTREE=<log2>x<threads>
  * (14x1024, 13x1024, 13x512 or 12x512) pins one.  This writes a text summary
- * of the classes seen, the device times and the choices into `out`; returns
- * the summary's full length.  No counterpart in the reference. */
+ * of the classes seen, the trial device times and the choices into `out`;
+ * returns the summary's full length.  No counterpart in the reference. */
+#define BENES_HIP_AUTOTUNE_TRIAL_CALLS 8
 int benes_hip_autotune_report(char* out, size_t len);
 
 /* Pins the subtree shape for later calls (one of the four above), or hands the

@@ -28,6 +28,18 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
+def test_header_constants_match_the_module():
+    with open(os.path.join(S.ROOT, "include", "benes_hip.h")) as f:
+        text = f.read()
+    trials = int(re.search(r"#define BENES_HIP_AUTOTUNE_TRIAL_CALLS (\d+)", text).group(1))
+    assert trials == B.AUTOTUNE_TRIAL_CALLS
+    assert trials % len(B.SUBTREE_SHAPES) == 0
+    codes = dict(re.findall(r"#define BENES_HIP_([A-Z_]+) (\d+)", text))
+    for name in ("OK", "INVALID_ARGUMENT", "OVERFLOW", "LENGTH", "LOGIC", "RUNTIME",
+                 "BUFFER_TOO_SMALL", "PENDING", "OUT_OF_RANGE"):
+        assert int(codes[name]) == getattr(B, name), name
+
+
 def test_layout_matches_oracle():
     for n in list(range(1, 130)) + [255, 4097, 65537, 1 << 20, (1 << 26), (1 << 32) + 12345,
                                     (1 << 40) + 3]:

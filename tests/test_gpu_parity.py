@@ -14,6 +14,7 @@ import hashlib
 import itertools
 import json
 import os
+import re
 
 import numpy as np
 import pytest
@@ -131,6 +132,30 @@ def test_every_subtree_shape(shape):
         assert hashlib.sha256(B.compress(perm)).hexdigest() == rec["sha256"]
     finally:
         B.set_subtree_shape(0, 0)
+
+
+def _tuner_line(cls: str) -> str:
+    return next((l for l in B.autotune_report().splitlines() if l.startswith(cls)), "")
+
+
+def test_tuner_trials_then_keeps_one_shape():
+    # A size class of its own (largest and total 2^21): its first calls cycle
+    # through every shape, then the choice is fixed; every call's bytes agree.
+    n = (1 << 21) + 4321
+    cls = "largest 2^21 total 2^21:"
+    perm = B.generate(B.FISHER_YATES, n, 2121)
+    want = S.oracle_compress(perm)
+    for _ in range(B.AUTOTUNE_TRIAL_CALLS):
+        if "->" in _tuner_line(cls):
+            break
+        assert B.compress(perm) == want
+    line = _tuner_line(cls)
+    assert "->" in line, line
+    times = [float(t) for t in re.findall(r"=([0-9.]+)ms", line)]
+    assert len(times) == len(B.SUBTREE_SHAPES) and all(t > 0 for t in times), line
+    for _ in range(3):
+        assert B.compress(perm) == want
+        assert _tuner_line(cls) == line
 
 
 def test_structured_inputs_large():
