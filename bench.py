@@ -62,13 +62,14 @@ def algorithmic_bytes(kernel: str, n: int, blob: int) -> float:
     figures times the entries one launch covers (one launch of a level
     kernel covers one depth of the whole permutation)."""
     per_launch = {
-        "k_union": 6.0 * n,             # inverse read (4 B/entry), slot parent words written (4 B/slot)
+        "k_hook_first": 6.0 * n,        # inverse read (4 B/entry), every slot's parent word written (4 B/slot)
+        "k_hook_second": 4.0 * n,       # inverse read
+        "k_union": 4.0 * n,             # inverse read
         "k_route_inputs": 10.0 * n + n / 8,   # P read, children written, slot words read, row bits
         "k_route_outputs": 8.0 * n + n / 4,   # I read, children written, row bits
         "k_level_lds": 16.0 * n + n / 4,      # P and I read, both children written, row bits
         "k_scatter_inverse": 8.0 * n,   # source read, inverse written
         "k_check_inverse": 4.0 * n,
-        "k_fill_roots": 4.0 * n,
         "k_pack": n / 8.0,              # one side's row bits of one depth, read and written
     }
     if kernel == "k_subtree":

@@ -421,14 +421,14 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     }
     const Launch per_entry = per_entry_list.commit(table);
     struct LevelPlan {
-        Launch describe, fill, hbm, lds, pack[2];
+        Launch describe, hbm, lds, pack[2];
         bool wide_uf = false;
     };
     std::vector<LevelPlan> levels(max_df);
     uint64_t max_desc = 1, max_rows = 1;
     for (uint32_t d = 0; d < max_df; ++d) {
         LevelPlan& lp = levels[d];
-        SegList describe, fill, hbm, lds, pack[2];
+        SegList describe, hbm, lds, pack[2];
         uint64_t desc_count = 0, row_count = 0;
         const uint64_t nodes = uint64_t{1} << d;
         for (uint32_t i = 0; i < count; ++i) {
@@ -441,7 +441,6 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             if (cmax > kLevelLds) {
                 const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
                 hbm.add(i, nodes * chunks, nodes, chunks, desc_count, row_count, row_words, true);
-                fill.add(i, ceil_div(jobs[i].n, kBlock * per_thread), 0, 0, 0, 0, 0, false);
                 lp.wide_uf = lp.wide_uf || maxpairs + 1 >= (uint64_t{1} << 31);
             } else {
                 lds.add(i, nodes, nodes, 1, desc_count, row_count, row_words, false);
@@ -462,7 +461,6 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             row_count += nodes * row_words;
         }
         lp.describe = describe.commit(table);
-        lp.fill = fill.commit(table);
         lp.hbm = hbm.commit(table);
         lp.lds = lds.commit(table);
         lp.pack[0] = pack[0].commit(table);
@@ -546,8 +544,10 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             using VI = std::remove_const_t<std::remove_pointer_t<decltype(inv)>>;
             using W = std::remove_pointer_t<decltype(uf)>;
             if (lp.hbm.count) {
-                BH_LAUNCH(prof, "k_fill_roots", (k_fill_roots<W>), grid(lp.fill), dim3(kBlock), 0,
-                          st, ditems, segp(lp.fill), lp.fill.count, uf);
+                BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W>), grid(lp.hbm),
+                          dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc, errs);
+                BH_LAUNCH(prof, "k_hook_second", (k_hook_second<VI, W>), grid(lp.hbm),
+                          dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc, errs);
                 BH_LAUNCH(prof, "k_union", (k_union<VI, W>), grid(lp.hbm), dim3(kBlock), 0, st,
                           segp(lp.hbm), lp.hbm.count, inv, uf, desc, errs);
                 BH_LAUNCH(prof, "k_route_inputs", (k_route_inputs<W>), grid(lp.hbm), dim3(kBlock),

@@ -17,7 +17,8 @@
 //   k_scatter_inverse  validation and the inverse permutation, no atomics
 // Per "global" level (nodes larger than one workgroup's LDS):
 //   k_describe       one thread per node: offsets and both sides' addresses
-//   k_union          one thread per output pair: parity union-find over the
+//   k_hook_first, k_hook_second, k_union
+//                    one thread per output pair: parity union-find over the
 //                    node's input slots in HBM (one CAS per hook)
 //   k_route_inputs   one thread per input pair: switch values, routing of P,
 //                    values ballot-packed into node-major bit rows
@@ -230,19 +231,6 @@ __global__ __launch_bounds__(kBlock) void k_check_inverse(const ItemCtx* __restr
     }
 }
 
-// Sets words [base, base + n) of each item to all-ones (union-find roots).
-template <typename W>
-__global__ __launch_bounds__(kBlock) void k_fill_roots(const ItemCtx* __restrict__ items,
-                                                       const Seg* __restrict__ segs,
-                                                       uint32_t nsegs, W* __restrict__ uf) {
-    const Seg& sg = find_seg(segs, nsegs);
-    const ItemCtx it = items[sg.item];
-    for (uint64_t i = (blockIdx.x - sg.block0) * kBlock + threadIdx.x; i < it.n;
-         i += sg.blocks * kBlock) {
-        uf[it.base + i] = UfWord<W>::root;
-    }
-}
-
 // Writes each item's header bytes (internal.hpp:431-454) at the start of its
 // zeroed blob, before any switch bit is OR'ed in.
 __global__ void k_headers(const ItemCtx* __restrict__ items, uint32_t count,
@@ -269,20 +257,28 @@ __global__ void k_headers(const ItemCtx* __restrict__ items, uint32_t count,
 // When an item has at least 8 nodes at a depth (its segment then starts on a
 // multiple of 8), blocks are renumbered so that every block of node j lands
 // on XCD j % 8, the node's union-find words, arrays and rows then living in
-// that one XCD's L2; consecutive blocks of one XCD visit its nodes in turn,
-// so the blocks in flight spread over all of them instead of crowding one
-// node's hottest words.
+// that one XCD's L2.  Blocks of one XCD then either take its nodes one after
+// another (`spread` false: the blocks in flight share one node, whose words
+// stay in L2), or visit its nodes in turn (`spread` true: the blocks in flight
+// cover all of them).  The union pass spreads: its compare-and-swaps go past
+// L2 to memory, and crowding them on one node's words serialises them there.
 constexpr uint32_t kXcds = 8;
 
-__device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uint64_t& chunk) {
+__device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uint64_t& chunk,
+                                              bool spread) {
     const uint64_t b = blockIdx.x - sg.block0;
     if (sg.nodes % kXcds != 0) {
         node = b / sg.chunks;
         chunk = b % sg.chunks;
     } else {
         const uint64_t xcd = b % kXcds, local = b / kXcds, per_xcd = sg.nodes / kXcds;
-        node = (local % per_xcd) * kXcds + xcd;
-        chunk = local / per_xcd;
+        if (spread) {
+            node = (local % per_xcd) * kXcds + xcd;
+            chunk = local / per_xcd;
+        } else {
+            node = (local / sg.chunks) * kXcds + xcd;
+            chunk = local % sg.chunks;
+        }
     }
     return node < sg.nodes && chunk < sg.chunks;
 }
@@ -311,7 +307,99 @@ __device__ __forceinline__ uint64_t in_slot(uint64_t x, uint64_t count) {
     return x >> 1;
 }
 
-// One thread per output pair: union of the two input slots that reach it.
+// The colouring's union-find runs in three passes over the output pairs.
+// Every input slot has exactly one even entry (2p for pair p, the even tail
+// entry count - 1 for slot 0 of an odd node), and each entry reaches one
+// output pair, whose edge joins the entry's slot to the other entry's slot.
+//   k_hook_first   each slot's word is written by the edge of its even entry:
+//                  the relation to the edge's other slot when that slot is
+//                  smaller, else the root word.  One plain store per slot, no
+//                  atomics; it replaces a fill of root words.
+//   k_hook_second  an edge reached by its larger slot's odd entry hooks that
+//                  slot under the smaller one when the first pass left it a
+//                  root (only this thread writes that word in this pass).
+//   k_union        the remaining edges: those reached by their larger slot's
+//                  odd entry whose larger slot was already hooked by the first
+//                  pass, i.e. slots larger than both neighbours (about a third
+//                  of the edges of a random permutation).  Both roots are found
+//                  and linked with the compare-and-swap hook.
+// Edges reached by their larger slot's even entry were settled by the first
+// pass (the larger slot was hooked through that very edge).  Every stored
+// word is a true parity relation to a smaller slot, so the forest invariants
+// of the union-find hold across the passes; a pass-three edge whose larger
+// slot already holds the edge's own relation (hooked by the second pass, or
+// rewritten to it by halving) is skipped.
+struct PairEdge {
+    uint64_t lo, hi;    // the two input slots, lo < hi (equal: a 2-cycle)
+    uint32_t label;     // value(lo) ^ value(hi)
+    bool hi_odd;        // the larger slot reaches this pair through its odd entry
+};
+
+template <typename VI>
+__device__ __forceinline__ PairEdge pair_edge(const VI* inv, uint64_t off, uint64_t cnt,
+                                              uint64_t q) {
+    const uint64_t a = static_cast<uint64_t>(inv[off + 2 * q]);
+    const uint64_t b = static_cast<uint64_t>(inv[off + 2 * q + 1]);
+    const uint64_t sa = in_slot(a, cnt), sb = in_slot(b, cnt);
+    PairEdge e;
+    e.lo = sa < sb ? sa : sb;
+    e.hi = sa < sb ? sb : sa;
+    e.label = static_cast<uint32_t>(1 ^ ((a ^ b) & 1));
+    e.hi_odd = ((sa < sb ? b : a) & 1) != 0;
+    return e;
+}
+
+template <typename VI, typename W>
+__global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ segs,
+                                                       uint32_t nsegs,
+                                                       const VI* __restrict__ inv,
+                                                       W* __restrict__ uf,
+                                                       const NodeDesc* __restrict__ desc,
+                                                       const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    uint64_t node, chunk;
+    if (!block_to_node(sg, node, chunk, false)) return;
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, cnt = nd.cnt, h = cnt >> 1;
+    const uint64_t q = chunk * kBlock + threadIdx.x;
+    W* u = uf + off;
+    if (q < h) {
+        const uint64_t a = static_cast<uint64_t>(inv[off + 2 * q]);
+        const uint64_t b = static_cast<uint64_t>(inv[off + 2 * q + 1]);
+        const uint64_t sa = in_slot(a, cnt), sb = in_slot(b, cnt);
+        const W label = static_cast<W>(1 ^ ((a ^ b) & 1));
+        if (!(a & 1)) u[sa] = sb < sa ? (static_cast<W>(sb) << 1) | label : UfWord<W>::root;
+        if (!(b & 1)) u[sb] = sa < sb ? (static_cast<W>(sa) << 1) | label : UfWord<W>::root;
+    } else if ((cnt & 1) && q == h) {
+        // The unpaired output entry: no edge; an even entry there leaves its
+        // slot a root.
+        const uint64_t x = static_cast<uint64_t>(inv[off + cnt - 1]);
+        if (!(x & 1)) u[in_slot(x, cnt)] = UfWord<W>::root;
+    }
+}
+
+template <typename VI, typename W>
+__global__ __launch_bounds__(kBlock) void k_hook_second(const Seg* __restrict__ segs,
+                                                        uint32_t nsegs,
+                                                        const VI* __restrict__ inv,
+                                                        W* __restrict__ uf,
+                                                        const NodeDesc* __restrict__ desc,
+                                                        const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    uint64_t node, chunk;
+    if (!block_to_node(sg, node, chunk, false)) return;
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, cnt = nd.cnt;
+    const uint64_t q = chunk * kBlock + threadIdx.x;
+    if (q >= (cnt >> 1)) return;
+    const PairEdge e = pair_edge(inv, off, cnt, q);
+    if (e.lo == e.hi || !e.hi_odd) return;
+    W* u = uf + off;
+    if (u[e.hi] == UfWord<W>::root) u[e.hi] = (static_cast<W>(e.lo) << 1) | e.label;
+}
+
 template <typename VI, typename W>
 __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, uint32_t nsegs,
                                                   const VI* __restrict__ inv, W* __restrict__ uf,
@@ -320,17 +408,19 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
     uint64_t node, chunk;
-    if (!block_to_node(sg, node, chunk)) return;
+    if (!block_to_node(sg, node, chunk, true)) return;
     const NodeDesc& nd = desc[sg.desc0 + node];
     const uint64_t off = nd.off, cnt = nd.cnt;
     const uint64_t q = chunk * kBlock + threadIdx.x;
     if (q >= (cnt >> 1)) return;
-    const uint64_t a = static_cast<uint64_t>(inv[off + 2 * q]);
-    const uint64_t b = static_cast<uint64_t>(inv[off + 2 * q + 1]);
-    const uint64_t sa = in_slot(a, cnt), sb = in_slot(b, cnt);
-    if (sa == sb) return;  // both entries of one input pair: a 2-cycle
-    uf_union_slots<DeviceForest>(uf + off, static_cast<W>(sa), static_cast<W>(sb),
-                                 static_cast<uint32_t>(1 ^ ((a ^ b) & 1)));
+    const PairEdge e = pair_edge(inv, off, cnt, q);
+    if (e.lo == e.hi || !e.hi_odd) return;
+    W* u = uf + off;
+    const W lo = static_cast<W>(e.lo), hi = static_cast<W>(e.hi);
+    const W w = DeviceForest::load(u + hi);
+    if (w == ((lo << 1) | e.label)) return;
+    if (w == UfWord<W>::root) uf_union_slots<DeviceForest>(u, lo, hi, e.label);
+    else uf_union_below<DeviceForest>(u, lo, w, e.label);
 }
 
 // Entry i (node-local index `off + i` in shared terms) of the level's
@@ -355,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restri
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
     uint64_t node, chunk;
-    if (!block_to_node(sg, node, chunk)) return;
+    if (!block_to_node(sg, node, chunk, false)) return;
     const ItemCtx it = items[sg.item];
     const NodeDesc& nd = desc[sg.desc0 + node];
     const uint64_t off = nd.off, cnt = nd.cnt, row = nd.row;
@@ -390,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict_
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
     uint64_t node, chunk;
-    if (!block_to_node(sg, node, chunk)) return;
+    if (!block_to_node(sg, node, chunk, false)) return;
     const NodeDesc& nd = desc[sg.desc0 + node];
     const uint64_t off = nd.off, cnt = nd.cnt, row = nd.row;
     const uint64_t h = cnt >> 1, lc = h + (cnt & 1);

@@ -155,6 +155,17 @@ __device__ __forceinline__ void uf_union_slots(W* uf, W a, W b, uint32_t label) 
     uf_link<F>(uf, rl, pl, rh, ph ^ static_cast<uint32_t>(found & 1), label);
 }
 
+// Union of slot `lo` with the component of a larger slot whose parent word
+// `w` (not the root word) has already been read: both roots are found and
+// linked, with no hook attempted on the larger slot itself.
+template <typename F, typename W>
+__device__ __forceinline__ void uf_union_below(W* uf, W lo, W w, uint32_t label) {
+    uint32_t pl, ph;
+    const W rl = uf_find<F>(uf, lo, pl);
+    const W rh = uf_find<F>(uf, w >> 1, ph);
+    uf_link<F>(uf, rl, pl, rh, ph ^ static_cast<uint32_t>(w & 1), label);
+}
+
 // Slot numbering inside a node of `count` entries (h = count/2 pairs).
 template <typename W>
 __device__ __forceinline__ W input_slot(uint64_t x, uint64_t count) {
