@@ -321,8 +321,8 @@ __device__ __forceinline__ uint64_t in_slot(uint64_t x, uint64_t count) {
 //   k_union        the remaining edges: those reached by their larger slot's
 //                  odd entry whose larger slot was already hooked by the first
 //                  pass, i.e. slots larger than both neighbours (about a third
-//                  of the edges of a random permutation).  Both roots are found
-//                  and linked with the compare-and-swap hook.
+//                  of the edges of a random permutation), joined by a Rem walk
+//                  with the compare-and-swap hook.
 // Edges reached by their larger slot's even entry were settled by the first
 // pass (the larger slot was hooked through that very edge).  Every stored
 // word is a true parity relation to a smaller slot, so the forest invariants
@@ -419,8 +419,10 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
     const W lo = static_cast<W>(e.lo), hi = static_cast<W>(e.hi);
     const W w = DeviceForest::load(u + hi);
     if (w == ((lo << 1) | e.label)) return;
-    if (w == UfWord<W>::root) uf_union_slots<DeviceForest>(u, lo, hi, e.label);
-    else uf_union_below<DeviceForest>(u, lo, w, e.label);
+    // w is never the root word after the first two passes; the general walk
+    // below would handle it.
+    if (w == UfWord<W>::root) uf_union_rem<DeviceForest>(u, hi, 0u, lo, 0u, e.label);
+    else uf_union_rem<DeviceForest>(u, w >> 1, static_cast<uint32_t>(w & 1), lo, 0u, e.label);
 }
 
 // Entry i (node-local index `off + i` in shared terms) of the level's

@@ -155,15 +155,38 @@ __device__ __forceinline__ void uf_union_slots(W* uf, W a, W b, uint32_t label) 
     uf_link<F>(uf, rl, pl, rh, ph ^ static_cast<uint32_t>(found & 1), label);
 }
 
-// Union of slot `lo` with the component of a larger slot whose parent word
-// `w` (not the root word) has already been read: both roots are found and
-// linked, with no hook attempted on the larger slot itself.
+// Union in the manner of Rem's algorithm: records value(x0) ^ value(y0) ==
+// label for two slots, starting from vertices x and y with parities px, py
+// to them.  The walk always climbs from the larger of the two current
+// vertices; it stops when they meet (already connected) or when the larger
+// one is a root, which is then hooked under the smaller current vertex (not
+// necessarily a root: any smaller vertex keeps parents below children, so
+// each tree stays rooted at its minimum slot).  Neither walk has to reach the
+// root of a large tree, so the words near big roots are read far less.  The
+// walk writes nothing but the hook: halving here costs more stores than the
+// later finds save.
 template <typename F, typename W>
-__device__ __forceinline__ void uf_union_below(W* uf, W lo, W w, uint32_t label) {
-    uint32_t pl, ph;
-    const W rl = uf_find<F>(uf, lo, pl);
-    const W rh = uf_find<F>(uf, w >> 1, ph);
-    uf_link<F>(uf, rl, pl, rh, ph ^ static_cast<uint32_t>(w & 1), label);
+__device__ __forceinline__ void uf_union_rem(W* uf, W x, uint32_t px, W y, uint32_t py,
+                                             uint32_t label) {
+    constexpr W kRoot = UfWord<W>::root;
+    while (x != y) {
+        if (x < y) {
+            const W t = x;
+            x = y;
+            y = t;
+            const uint32_t tp = px;
+            px = py;
+            py = tp;
+        }
+        W w = F::load(uf + x);
+        if (w == kRoot) {
+            const W found = F::hook(uf + x, (y << 1) | static_cast<W>((px ^ py ^ label) & 1));
+            if (found == kRoot) return;
+            w = found;
+        }
+        px ^= static_cast<uint32_t>(w & 1);
+        x = w >> 1;
+    }
 }
 
 // Slot numbering inside a node of `count` entries (h = count/2 pairs).
