@@ -877,13 +877,28 @@ __device__ __forceinline__ uint64_t middle_from(const Plan& P, uint64_t mid, uin
     return mid;
 }
 
+// Bits [pos, pos + len) (len <= 64) of an LDS bit array.
+__device__ __forceinline__ uint64_t lds_bits(const uint64_t* bits, uint32_t pos, uint32_t len) {
+    const uint32_t w = pos >> 6, sh = pos & 63;
+    uint64_t v = bits[w] >> sh;
+    if (sh + len > 64) v |= bits[w + 1] << (64 - sh);
+    return len >= 64 ? v : v & ((uint64_t{1} << len) - 1);
+}
+
 // Writes the switch bits of every internal node at relative level k of a
 // subtree (absolute depth d0 + k) into the blob.  Bits are produced packet by
 // packet: within a cluster group, one packet's share of a level is the
 // concatenation of that packet's pair range from each node in path order
 // (internal.hpp:604-649), so a run of consecutive nodes is one contiguous
-// stretch of the blob, written as whole words.  `sbits`/`tbits` hold the
-// level's switch values at the position of each node's pair in LDS.
+// stretch of the blob.  `sbits`/`tbits` hold the level's switch values at the
+// position of each node's pair in LDS.
+//
+// Jobs are (set of nodes sharing a cluster group, packet).  When every node
+// of the set fills its whole share of the packet (all but the last packet of
+// a set, and sets without leaves), the run is `set_nodes` slices of one span
+// each: one lane per output word gathers its word's slices from LDS and
+// stores it, or ORs it in when the word is shared with a neighbouring run.
+// The other jobs place nodes of differing shares with a prefix sum.
 __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot& root,
                                                 uint32_t d0, uint64_t b0, uint32_t n0,
                                                 uint32_t k, const uint64_t* sbits,
@@ -899,22 +914,68 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
         const uint32_t r = sd.relative_level, g = sd.group_depth;
         const uint32_t R = d - r;
         const uint32_t span_log2 = g - r - 1;
-        const uint64_t span = uint64_t{1} << span_log2;
+        const uint32_t span = 1u << span_log2;
         const uint64_t* rows = side == 0 ? sbits : tbits;
         const uint64_t secbase = (side == 0 ? P.input_offset : P.output_offset) * 8;
         // Sets of consecutive nodes that share one cluster group.
         const bool one_set = R <= d0;
         const uint32_t nsets = one_set ? 1u : (1u << (R - d0));
         const uint32_t set_nodes = one_set ? (1u << k) : (1u << r);
-        const uint32_t packets = static_cast<uint32_t>((maxpairs + span - 1) >> span_log2);
+        const uint32_t packets = (maxpairs + span - 1) >> span_log2;
         const uint32_t jobs = nsets * packets;
-        if ((static_cast<uint64_t>(set_nodes) << span_log2) <= 64) {
-            // Short runs: one lane per (set, packet).  Within a cluster group
-            // the nodes' shares of one packet follow each other in path
-            // order, so the run starts at the first internal node's address
-            // and each later node continues where the previous one ended.
+        const uint32_t len = set_nodes << span_log2;
+        auto full_share = [&](uint32_t set, uint32_t first_pair) {
+            const uint32_t minc = one_set ? (n0 >> k) : (sub_count(n0, R - d0, set) >> r);
+            return minc > L && (minc >> 1) >= first_pair + span;
+        };
+        // Equal shares: one lane per output word.
+        const uint32_t wpj = (len + 63) / 64 + 1;  // words one run can touch
+        {
+            uint32_t cached_set = ~0u;
+            SideAddr a{};
+            for (uint32_t t = threadIdx.x; t < jobs * wpj; t += blockDim.x) {
+                const uint32_t job = t / wpj, wi = t - job * wpj;
+                const uint32_t set = job / packets, pk = job - set * packets;
+                const uint32_t first_pair = pk << span_log2;
+                if (!full_share(set, first_pair)) continue;
+                const uint32_t first = set * set_nodes;
+                if (set != cached_set) {
+                    const uint64_t path = (b0 << k) | first;
+                    a = address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
+                    cached_set = set;
+                }
+                const uint64_t dst0 = secbase + bit_of(a, first_pair);
+                const uint64_t ow = (dst0 >> 6) + wi, wbit = ow << 6;
+                if (wbit >= dst0 + len) continue;
+                const uint32_t o_lo = wbit > dst0 ? static_cast<uint32_t>(wbit - dst0) : 0u;
+                const uint32_t o_hi =
+                    static_cast<uint32_t>(min(static_cast<uint64_t>(len), wbit + 64 - dst0));
+                uint32_t sh = static_cast<uint32_t>(dst0 + o_lo - wbit);
+                uint64_t v = 0;
+                for (uint32_t o = o_lo; o < o_hi;) {
+                    const uint32_t in_node = o & (span - 1);
+                    const uint32_t take = min(o_hi - o, span - in_node);
+                    v |= lds_bits(rows, noff[first + (o >> span_log2)] + first_pair + in_node, take)
+                         << sh;
+                    sh += take;
+                    o += take;
+                }
+                if (o_hi - o_lo == 64) {
+                    blob[ow] = v;
+                } else if (v) {
+                    atomicOr(reinterpret_cast<unsigned long long*>(blob + ow),
+                             static_cast<unsigned long long>(v));
+                }
+            }
+        }
+        if (len <= 64) {
+            // Differing shares, short runs: one lane per (set, packet); the run
+            // starts at the first internal node's address and each later node
+            // continues where the previous one ended.
             for (uint32_t job = threadIdx.x; job < jobs; job += blockDim.x) {
                 const uint32_t set = job / packets, pk = job - set * packets;
+                const uint32_t first_pair = pk << span_log2;
+                if (full_share(set, first_pair)) continue;
                 const uint32_t first = set * set_nodes, end = first + set_nodes;
                 uint32_t f = first;
                 while (f < end && sub_count(n0, k, f) <= L) ++f;
@@ -922,77 +983,44 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
                 const uint64_t path = (b0 << k) | f;
                 const SideAddr a =
                     address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
-                const uint32_t first_pair = pk << span_log2;
                 uint64_t dst = secbase + bit_of(a, first_pair);
                 for (uint32_t pi = f; pi < end; ++pi) {
                     const uint32_t share = static_cast<uint32_t>(
                         packet_share(sub_count(n0, k, pi), pk, span_log2, L));
-                    const uint32_t pos0 = noff[pi] + first_pair;
-                    lane_or_bits(blob, dst, share, [&](uint64_t b) {
-                        const uint32_t pos = pos0 + static_cast<uint32_t>(b);
-                        return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
-                    });
+                    if (share) or_word(blob, dst, lds_bits(rows, noff[pi] + first_pair, share));
                     dst += share;
                 }
             }
             continue;
         }
-        SideAddr cached{};
-        uint32_t cached_set = ~0u;
-        // Each wave takes a contiguous range of jobs, so it revisits one set.
-        const uint32_t per_wave = (jobs + nwaves - 1) / nwaves;
-        const uint32_t job_end = min(jobs, (wave + 1) * per_wave);
-        for (uint32_t job = wave * per_wave; job < job_end; ++job) {
+        // Differing shares, long runs: lanes take nodes, a wave prefix sum
+        // places them.
+        for (uint32_t job = wave; job < jobs; job += nwaves) {
             const uint32_t set = job / packets, pk = job - set * packets;
-            const uint32_t first = set * set_nodes;
-            const uint32_t minc = one_set ? (n0 >> k) : (sub_count(n0, R - d0, set) >> r);
-            const uint64_t first_pair = static_cast<uint64_t>(pk) << span_log2;
-            if (minc > L && (minc >> 1) >= first_pair + span) {
-                // Every node fills its share of this packet: one run.  The
-                // set's first node's address serves every packet of the set.
-                if (set != cached_set) {
-                    const uint64_t path = (b0 << k) | first;
-                    cached = address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d,
-                                        path);
-                    cached_set = set;
+            const uint32_t first_pair = pk << span_log2;
+            if (full_share(set, first_pair)) continue;  // wave-uniform
+            const uint32_t first = set * set_nodes, end = first + set_nodes;
+            uint32_t f = first;
+            while (f < end && sub_count(n0, k, f) <= L) ++f;  // wave-uniform
+            if (f == end) continue;
+            const uint64_t path = (b0 << k) | f;
+            const SideAddr a =
+                address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
+            const uint64_t run0 = secbase + bit_of(a, first_pair);
+            uint64_t carry = 0;
+            for (uint32_t base = f; base < end; base += 64) {
+                const uint32_t pi = base + lane_id();
+                const uint64_t share =
+                    pi < end ? packet_share(sub_count(n0, k, pi), pk, span_log2, L) : 0;
+                const uint64_t incl = wave_inclusive_sum(share);
+                if (share) {
+                    const uint32_t pos0 = noff[pi] + first_pair;
+                    lane_or_bits(blob, run0 + carry + incl - share, share, [&](uint64_t b) {
+                        const uint32_t pos = pos0 + static_cast<uint32_t>(b);
+                        return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
+                    });
                 }
-                const SideAddr& a = cached;
-                wave_write_run(blob, secbase + bit_of(a, first_pair),
-                               static_cast<uint64_t>(set_nodes) << span_log2, [&](uint64_t q) {
-                                   const uint32_t pos =
-                                       noff[first + static_cast<uint32_t>(q >> span_log2)] +
-                                       static_cast<uint32_t>(first_pair + (q & (span - 1)));
-                                   return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
-                               });
-            } else {
-                // Node shares of this packet differ (its last packet, or a
-                // level mixing leaves and internal nodes): the run starts at
-                // the set's first internal node and every node continues
-                // where the previous one ended; lanes take nodes, a wave
-                // prefix sum places them.
-                const uint32_t end = first + set_nodes;
-                uint32_t f = first;
-                while (f < end && sub_count(n0, k, f) <= L) ++f;  // wave-uniform
-                if (f == end) continue;
-                const uint64_t path = (b0 << k) | f;
-                const SideAddr a = address_at(
-                    P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
-                const uint64_t run0 = secbase + bit_of(a, first_pair);
-                uint64_t carry = 0;
-                for (uint32_t base = f; base < end; base += 64) {
-                    const uint32_t pi = base + lane_id();
-                    const uint64_t share =
-                        pi < end ? packet_share(sub_count(n0, k, pi), pk, span_log2, L) : 0;
-                    const uint64_t incl = wave_inclusive_sum(share);
-                    if (share) {
-                        const uint32_t pos0 = noff[pi] + static_cast<uint32_t>(first_pair);
-                        lane_or_bits(blob, run0 + carry + incl - share, share, [&](uint64_t b) {
-                            const uint32_t pos = pos0 + static_cast<uint32_t>(b);
-                            return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
-                        });
-                    }
-                    carry += __shfl(incl, 63);
-                }
+                carry += __shfl(incl, 63);
             }
         }
     }
