@@ -337,8 +337,7 @@ struct SegList {
     std::vector<Seg> segs;
     uint64_t blocks = 0;
     void add(uint32_t item, uint64_t blocks_of_item, uint64_t nodes, uint64_t chunks,
-             uint64_t desc0, uint64_t row0, uint64_t row_words, bool xcd_align,
-             bool by_lane = false) {
+             uint64_t desc0, uint64_t row0, uint64_t row_words, bool xcd_align) {
         if (blocks_of_item == 0) return;
         if (xcd_align) {
             blocks = round8(blocks);
@@ -353,7 +352,6 @@ struct SegList {
         s.row0 = row0;
         s.row_words = row_words;
         s.item = item;
-        s.lanes = by_lane ? 1u : 0u;
         segs.push_back(s);
         blocks += blocks_of_item;
     }
@@ -451,11 +449,10 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                 const uint32_t span_log2 = sd.group_depth - sd.relative_level - 1;
                 const uint64_t packets = ceil_div(maxpairs, uint64_t{1} << span_log2);
                 const uint64_t groups = uint64_t{1} << (d - sd.relative_level);
-                // A lane per job when a full packet's share of the level fits
-                // one word (2^(group depth - 1) bits), else a wave per job.
-                const bool by_lane = sd.group_depth <= 7;
-                pack[side].add(i, ceil_div(groups * packets, by_lane ? kBlock : kBlock / 64),
-                               nodes, packets, desc_count, row_count, row_words, false, by_lane);
+                pack[side].add(i,
+                               ceil_div(groups * packets * pack_words_per_job(sd.group_depth),
+                                        kBlock),
+                               nodes, packets, desc_count, row_count, row_words, false);
             }
             desc_count += nodes;
             row_count += nodes * row_words;

@@ -74,7 +74,7 @@ struct Seg {
     uint64_t row0;       // first row word of the item at this depth
     uint64_t row_words;  // row words per node
     uint32_t item;
-    uint32_t lanes;      // k_pack: one lane (1) or one wave (0) per job
+    uint32_t pad;
 };
 
 // Segment of the block: the last one whose first block is <= blockIdx.x.
@@ -155,32 +155,6 @@ __device__ __forceinline__ void lane_or_row(uint64_t* blob, uint64_t dst, const 
     for (uint64_t b = 0; b < len; b += 64) {
         const uint32_t cnt = static_cast<uint32_t>(min(uint64_t{64}, len - b));
         or_word(blob, dst + b, row_range(rows, row, first + b, cnt));
-    }
-}
-
-// Copies `len` bits, bit k given by get(k), to blob bits [dst0, dst0 + len).
-// Words wholly inside the run are stored; the (at most two) edge words are
-// OR'ed atomically because neighbouring runs share them.
-template <typename Get>
-__device__ __forceinline__ void wave_write_run(uint64_t* blob, uint64_t dst0, uint64_t len,
-                                               Get get) {
-    if (len == 0) return;
-    const uint32_t lane = lane_id();
-    const uint64_t w_first = dst0 >> 6, w_last = (dst0 + len - 1) >> 6;
-    for (uint64_t w = w_first; w <= w_last; ++w) {
-        const uint64_t o = (w << 6) + lane;
-        const bool valid = o >= dst0 && o < dst0 + len;
-        const bool b = valid && get(o - dst0);
-        const uint64_t val = __ballot(b);
-        const uint64_t vm = __ballot(valid);
-        if (lane == 0) {
-            if (vm == ~uint64_t{0}) {
-                blob[w] = val;
-            } else if (val) {
-                atomicOr(reinterpret_cast<unsigned long long*>(blob + w),
-                         static_cast<unsigned long long>(val));
-            }
-        }
     }
 }
 
@@ -580,10 +554,20 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
     }
 }
 
-// One job per (cluster group, packet) of one item at depth d on one side:
-// a wave per job, or a lane per job when a full packet's share of the level
-// (2^(group depth - 1) bits) fits one word.  In the packet where node shares
-// differ, lanes take the nodes, each placing its share from its own address.
+// Words of the blob one packet's share of a level (2^(g-1) bits, g the
+// cluster group's depth) can touch.
+__host__ __device__ __forceinline__ uint64_t pack_words_per_job(uint32_t g) {
+    return ((uint64_t{1} << (g - 1)) + 63) / 64 + 1;
+}
+
+// Jobs are (cluster group, packet) of one item at depth d on one side, and
+// each job has pack_words_per_job(g) threads.  When every node of the group's
+// level fills its share of the packet, the share is one run of 2^r slices of
+// one span each (internal.hpp:604-649): each thread gathers one output word
+// of it from the nodes' bit rows and stores it, OR-ing only words shared with
+// neighbouring runs.  In the packet where node shares differ, the job's
+// threads take the nodes in turn, each placing its node's share at the node's
+// own address.
 __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ items,
                                                  const Seg* __restrict__ segs, uint32_t nsegs,
                                                  int side, uint32_t d,
@@ -598,57 +582,52 @@ __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ ite
     uint64_t* blob = blob_all + it.blob;
     const NodeDesc* dsc = desc + sg.desc0;
     const uint64_t packets = sg.chunks;
-    const bool by_lane = sg.lanes != 0;
-    const uint64_t job = by_lane ? (blockIdx.x - sg.block0) * kBlock + threadIdx.x
-                                 : (blockIdx.x - sg.block0) * (kBlock / 64) + (threadIdx.x >> 6);
     const SideDepth sd = P.side[side][d];
     const uint32_t r = sd.relative_level, g = sd.group_depth;
+    const uint64_t wpj = pack_words_per_job(g);
+    const uint64_t t = (blockIdx.x - sg.block0) * kBlock + threadIdx.x;
+    const uint64_t job = t / wpj, wi = t - job * wpj;
     const uint32_t root_depth = d - r;
     const uint64_t groups = uint64_t{1} << root_depth;
     const uint64_t group = job / packets, packet = job % packets;
-    if (group >= groups) return;  // whole waves in wave mode; lanes in lane mode
+    if (group >= groups) return;
     const uint32_t span_log2 = g - r - 1;
+    const uint64_t span = uint64_t{1} << span_log2;
     const uint64_t C = node_count(P.n, root_depth, group);
     const uint64_t first_pair = packet << span_log2;
     const uint64_t min_pairs = (C >> r) >> 1;
     const uint64_t b0 = group << r;
     const uint64_t nodes = uint64_t{1} << r;
     const uint64_t secbase = (side == 0 ? P.input_offset : P.output_offset) * 8;
-    const uint64_t mask = (uint64_t{1} << span_log2) - 1;
-    const bool full = min_pairs >= first_pair + (uint64_t{1} << span_log2);
-    if (by_lane) {
-        if (full) {
-            // nodes << span_log2 <= 64 bits: node j's span of pairs at bit j * span.
-            uint64_t acc = 0;
-            for (uint64_t pi = 0; pi < nodes; ++pi) {
-                acc |= row_range(rows, dsc[b0 + pi].row, first_pair, 1u << span_log2)
-                       << (pi << span_log2);
-            }
-            or_word(blob, secbase + bit_of(dsc[b0].a[side], first_pair), acc);
-        } else {
-            for (uint64_t pi = 0; pi < nodes; ++pi) {
-                const NodeDesc& nd = dsc[b0 + pi];
-                lane_or_row(blob, secbase + bit_of(nd.a[side], first_pair), rows, nd.row,
-                            first_pair, packet_share(nd.cnt, packet, span_log2, P.leaf));
-            }
+    if ((C >> r) > P.leaf && min_pairs >= first_pair + span) {
+        const uint64_t len = nodes << span_log2;
+        const uint64_t dst0 = secbase + bit_of(dsc[b0].a[side], first_pair);
+        const uint64_t ow = (dst0 >> 6) + wi, wbit = ow << 6;
+        if (wbit >= dst0 + len) return;
+        const uint64_t o_lo = wbit > dst0 ? wbit - dst0 : 0;
+        const uint64_t o_hi = min(len, wbit + 64 - dst0);
+        uint32_t sh = static_cast<uint32_t>(dst0 + o_lo - wbit);
+        uint64_t v = 0;
+        for (uint64_t o = o_lo; o < o_hi;) {
+            const uint64_t in_node = o & (span - 1);
+            const uint32_t take = static_cast<uint32_t>(min(o_hi - o, span - in_node));
+            const uint64_t row = sg.row0 + (b0 + (o >> span_log2)) * sg.row_words;
+            v |= row_range(rows, row, first_pair + in_node, take) << sh;
+            sh += take;
+            o += take;
+        }
+        if (o_hi - o_lo == 64) {
+            blob[ow] = v;
+        } else if (v) {
+            atomicOr(reinterpret_cast<unsigned long long*>(blob + ow),
+                     static_cast<unsigned long long>(v));
         }
         return;
     }
-    if (full) {
-        // Every node of the level fills its share of this packet: one run.
-        wave_write_run(blob, secbase + bit_of(dsc[b0].a[side], first_pair), nodes << span_log2,
-                       [&](uint64_t k) {
-                           const NodeDesc& nd = dsc[b0 + (k >> span_log2)];
-                           return row_bit(rows, nd.row, first_pair + (k & mask)) != 0;
-                       });
-    } else {
-        for (uint64_t base = 0; base < nodes; base += 64) {
-            const uint64_t pi = base + lane_id();
-            if (pi >= nodes) continue;
-            const NodeDesc& nd = dsc[b0 + pi];
-            lane_or_row(blob, secbase + bit_of(nd.a[side], first_pair), rows, nd.row, first_pair,
-                        packet_share(nd.cnt, packet, span_log2, P.leaf));
-        }
+    for (uint64_t pi = wi; pi < nodes; pi += wpj) {
+        const NodeDesc& nd = dsc[b0 + pi];
+        lane_or_row(blob, secbase + bit_of(nd.a[side], first_pair), rows, nd.row, first_pair,
+                    packet_share(nd.cnt, packet, span_log2, P.leaf));
     }
 }
 
