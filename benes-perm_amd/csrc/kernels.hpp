@@ -879,6 +879,7 @@ __device__ __forceinline__ uint64_t lds_bits(const uint64_t* bits, uint32_t pos,
 // stores it, or ORs it in when the word is shared with a neighbouring run.
 // The other jobs place nodes of differing shares with a prefix sum.
 __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot& root,
+                                                const SideAddr (*first_addr)[kSubLog2Max + 1],
                                                 uint32_t d0, uint64_t b0, uint32_t n0,
                                                 uint32_t k, const uint64_t* sbits,
                                                 const uint64_t* tbits, const uint16_t* noff,
@@ -920,7 +921,9 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
                 const uint32_t first = set * set_nodes;
                 if (set != cached_set) {
                     const uint64_t path = (b0 << k) | first;
-                    a = address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
+                    a = first == 0 ? first_addr[side][k]
+                                   : address_at(P, walk_from(P, side, root.walk[side], d0, d, path),
+                                                d, path);
                     cached_set = set;
                 }
                 const uint64_t dst0 = secbase + bit_of(a, first_pair);
@@ -929,15 +932,21 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
                 const uint32_t o_lo = wbit > dst0 ? static_cast<uint32_t>(wbit - dst0) : 0u;
                 const uint32_t o_hi =
                     static_cast<uint32_t>(min(static_cast<uint64_t>(len), wbit + 64 - dst0));
-                uint32_t sh = static_cast<uint32_t>(dst0 + o_lo - wbit);
+                const uint32_t sh0 = static_cast<uint32_t>(dst0 + o_lo - wbit);
+                // The word's slices are visited starting at a lane-dependent
+                // one: lanes of a wave hold consecutive words, whose slices
+                // lie one span of nodes apart in LDS, and in step they would
+                // all hit the same bank.
+                const uint32_t s_lo = o_lo >> span_log2, ns = ((o_hi - 1) >> span_log2) - s_lo + 1;
+                uint32_t j = lane_id() % ns;
                 uint64_t v = 0;
-                for (uint32_t o = o_lo; o < o_hi;) {
-                    const uint32_t in_node = o & (span - 1);
-                    const uint32_t take = min(o_hi - o, span - in_node);
-                    v |= lds_bits(rows, noff[first + (o >> span_log2)] + first_pair + in_node, take)
-                         << sh;
-                    sh += take;
-                    o += take;
+                for (uint32_t it = 0; it < ns; ++it) {
+                    const uint32_t sl = s_lo + j;
+                    const uint32_t lo = max(o_lo, sl << span_log2);
+                    const uint32_t hi = min(o_hi, (sl + 1) << span_log2);
+                    v |= lds_bits(rows, noff[first + sl] + first_pair + (lo & (span - 1)), hi - lo)
+                         << (sh0 + lo - o_lo);
+                    j = j + 1 == ns ? 0 : j + 1;
                 }
                 if (o_hi - o_lo == 64) {
                     blob[ow] = v;
@@ -1032,10 +1041,21 @@ __global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__
     const uint64_t base = node_offset(P.n, d0, b0);
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nthreads = blockDim.x, nwaves = blockDim.x >> 6;
-    if (tid == 0) {
-        root.walk[0] = walk_to(P, 0, d0, b0);
-        root.walk[1] = walk_to(P, 1, d0, b0);
-        root.middle = middle_from(P, 0, 0, d0, b0);
+    // Walks to the subtree root, and the address of the first node of every
+    // internal level on both sides, computed by different lanes at once:
+    // each is a chain of dependent table reads, and the emission of a level
+    // would otherwise wait for one.
+    __shared__ SideAddr first_addr[2][kSubLog2Max + 1];
+    if (tid == 0) root.walk[0] = walk_to(P, 0, d0, b0);
+    if (tid == 1) root.walk[1] = walk_to(P, 1, d0, b0);
+    if (tid == 2) root.middle = middle_from(P, 0, 0, d0, b0);
+    if (tid >= 64 && tid < 64 + 2 * (kSubLog2Max + 1)) {
+        const uint32_t t = tid - 64, side = t & 1, k = t >> 1;
+        if (((n0 + (1u << k) - 1) >> k) > L) {
+            const uint32_t d = d0 + k;
+            const uint64_t path = b0 << k;
+            first_addr[side][k] = address_at(P, walk_to(P, side, d, path), d, path);
+        }
     }
     // Depth d0 - 1 wrote level (d0 - 1) & 1; depth 0 reads the item's source.
     const uint32_t* plev = ((d0 - 1) & 1) ? lev1 : lev0;
@@ -1201,7 +1221,7 @@ __global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__
             }
         }
         __syncthreads();
-        emit_level_runs(P, root, d0, b0, n0, k, sbits, tbits, noff, blob);
+        emit_level_runs(P, root, first_addr, d0, b0, n0, k, sbits, tbits, noff, blob);
         __syncthreads();
         cur ^= 1;
     }
