@@ -1116,28 +1116,31 @@ __global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__
             next[off + perm[off + x]] = static_cast<uint16_t>(x);
         }
         __syncthreads();
+        // Pair-indexed steps run over a flat index g = node * S + pair, S the
+        // power of two covering a node's pairs (and the odd tail slot when
+        // some node is odd): a wave covers 64 / S small nodes at once, or 64
+        // pairs of one large node.
+        const uint32_t odd_any = (q & 1u) | (m != 0 ? 1u : 0u);
+        const uint32_t s_log2 = 32u - __builtin_clz(((cmax >> 1) + odd_any - 1) | 1u);
+        const uint32_t S = 1u << s_log2, total = nodes << s_log2;
         // Union of the two input slots reaching each output pair (the same
-        // input-slot forest as the HBM levels).  Items are (node, 64-pair
-        // chunk).
-        // Each lane takes kUnionBatch pairs at a time and issues their first
-        // hooks together (most settle there), so the LDS round trips of
-        // independent unions overlap.
-        const uint32_t ppn = ((cmax >> 1) + 1 + 63) >> 6;
+        // input-slot forest as the HBM levels).  Each lane takes kUnionBatch
+        // pairs at a time and issues their first hooks together (most settle
+        // there), so the LDS round trips of independent unions overlap.
         constexpr uint32_t kUnionBatch = 4;
-        for (uint32_t item0 = wave; item0 < nodes * ppn; item0 += kUnionBatch * nwaves) {
+        for (uint32_t g0 = tid; g0 < total; g0 += kUnionBatch * nthreads) {
             uint32_t base[kUnionBatch], hi[kUnionBatch], lo[kUnionBatch], found[kUnionBatch];
             bool act[kUnionBatch];
 #pragma unroll
             for (uint32_t u = 0; u < kUnionBatch; ++u) {
-                const uint32_t item = item0 + u * nwaves;
+                const uint32_t g = g0 + u * nthreads;
                 act[u] = false;
-                if (item >= nodes * ppn) continue;
-                const uint32_t pi = item / ppn, chunk = item - pi * ppn;
+                if (g >= total) continue;
+                const uint32_t pi = g >> s_log2, qq = g & (S - 1);
                 const uint32_t c = sub_count(n0, k, pi);
-                const uint32_t q = chunk * 64 + lane;
-                if (c <= L || q >= (c >> 1)) continue;
+                if (c <= L || qq >= (c >> 1)) continue;
                 const uint32_t off = noff[pi];
-                const uint32_t a = next[off + 2 * q], b = next[off + 2 * q + 1];
+                const uint32_t a = next[off + 2 * qq], b = next[off + 2 * qq + 1];
                 const uint32_t sa = input_slot<uint32_t>(a, c), sb = input_slot<uint32_t>(b, c);
                 if (sa == sb) continue;
                 act[u] = true;
@@ -1161,61 +1164,73 @@ __global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__
             }
         }
         __syncthreads();
-        // Input switch values: each pair's parity to its root.
-        for (uint32_t item = wave; item < nodes * ppn; item += nwaves) {
-            const uint32_t pi = item / ppn, chunk = item - pi * ppn;
-            const uint32_t c = sub_count(n0, k, pi);
-            if (c <= L) continue;  // wave-uniform
-            const uint32_t off = noff[pi], h = c >> 1;
-            const uint32_t p = chunk * 64 + lane;
-            uint32_t s = 0;
-            if (p < h) uf_find<GroupForest>(uf + off, input_pair_slot<uint32_t>(p, c), s);
-            const uint64_t ms = __ballot(s != 0);
-            if (lane == 0 && ms) {
-                const uint32_t pos = off + chunk * 64, w = pos >> 6, sh = pos & 63;
-                atomicOr(reinterpret_cast<unsigned long long*>(sbits + w), ms << sh);
-                if (sh) atomicOr(reinterpret_cast<unsigned long long*>(sbits + w + 1),
-                                 ms >> (64 - sh));
+        // ORs a wave's ballot of per-pair bits into an LDS bit array at each
+        // node's pair positions: one 64-pair chunk of one node (S >= 64), or
+        // 64 / S whole nodes, each by the lane holding its first pair.
+        auto or_ballot = [&](uint64_t* bits, uint64_t mask, uint32_t pi, uint32_t qq) {
+            uint64_t seg;
+            uint32_t pos;
+            if (S >= 64) {
+                if (lane != 0) return;
+                seg = mask;
+                pos = noff[pi] + qq;
+            } else {
+                if (qq != 0 || pi >= nodes) return;
+                seg = (mask >> lane) & ((uint64_t{1} << S) - 1);
+                pos = noff[pi];
             }
+            if (!seg) return;
+            const uint32_t w = pos >> 6, sh = pos & 63;
+            atomicOr(reinterpret_cast<unsigned long long*>(bits + w), seg << sh);
+            if (sh && (seg >> (64 - sh))) {
+                atomicOr(reinterpret_cast<unsigned long long*>(bits + w + 1), seg >> (64 - sh));
+            }
+        };
+        // Input switch values: each pair's parity to its root.
+        for (uint32_t gb = wave * 64; gb < total; gb += nthreads) {
+            const uint32_t g = gb + lane;
+            const uint32_t pi = g >> s_log2, qq = g & (S - 1);
+            uint32_t sv = 0;
+            if (g < total) {
+                const uint32_t c = sub_count(n0, k, pi);
+                if (c > L && qq < (c >> 1)) {
+                    uf_find<GroupForest>(uf + noff[pi], input_pair_slot<uint32_t>(qq, c), sv);
+                }
+            }
+            or_ballot(sbits, __ballot(sv != 0), pi, qq);
         }
         __syncthreads();
         // Output switch values t_q = colour(I[2q]), colour(x) = s_(x>>1) ^
         // (x & 1) and 0 for the odd tail; then the children of P.
-        for (uint32_t item = wave; item < nodes * ppn; item += nwaves) {
-            const uint32_t pi = item / ppn, chunk = item - pi * ppn;
-            const uint32_t c = sub_count(n0, k, pi);
-            if (c <= L) continue;  // wave-uniform
-            const uint32_t off = noff[pi], h = c >> 1;
-            const uint32_t q = chunk * 64 + lane;
+        for (uint32_t gb = wave * 64; gb < total; gb += nthreads) {
+            const uint32_t g = gb + lane;
+            const uint32_t pi = g >> s_log2, qq = g & (S - 1);
             uint32_t t = 0;
-            if (q < h) {
-                const uint32_t a0 = next[off + 2 * q];
-                if (!((c & 1) && a0 == c - 1)) {
-                    const uint32_t pos = off + (a0 >> 1);
-                    t = static_cast<uint32_t>((sbits[pos >> 6] >> (pos & 63)) & 1) ^ (a0 & 1u);
+            if (g < total) {
+                const uint32_t c = sub_count(n0, k, pi);
+                if (c > L && qq < (c >> 1)) {
+                    const uint32_t off = noff[pi];
+                    const uint32_t a0 = next[off + 2 * qq];
+                    if (!((c & 1) && a0 == c - 1)) {
+                        const uint32_t pos = off + (a0 >> 1);
+                        t = static_cast<uint32_t>((sbits[pos >> 6] >> (pos & 63)) & 1) ^ (a0 & 1u);
+                    }
                 }
             }
-            const uint64_t mt = __ballot(t != 0);
-            if (lane == 0 && mt) {
-                const uint32_t pos = off + chunk * 64, w = pos >> 6, sh = pos & 63;
-                atomicOr(reinterpret_cast<unsigned long long*>(tbits + w), mt << sh);
-                if (sh) atomicOr(reinterpret_cast<unsigned long long*>(tbits + w + 1),
-                                 mt >> (64 - sh));
-            }
+            or_ballot(tbits, __ballot(t != 0), pi, qq);
         }
         __syncthreads();
-        for (uint32_t item = wave; item < nodes * ppn; item += nwaves) {
-            const uint32_t pi = item / ppn, chunk = item - pi * ppn;
+        for (uint32_t g = tid; g < total; g += nthreads) {
+            const uint32_t pi = g >> s_log2, p = g & (S - 1);
             const uint32_t c = sub_count(n0, k, pi);
             if (c <= L) continue;
             const uint32_t off = noff[pi], h = c >> 1, lc = h + (c & 1);
-            const uint32_t p = chunk * 64 + lane;
             if (p < h) {
                 const uint32_t pos = off + p;
-                const bool s = ((sbits[pos >> 6] >> (pos & 63)) & 1) != 0;
+                const bool sv = ((sbits[pos >> 6] >> (pos & 63)) & 1) != 0;
                 const uint32_t a0 = perm[off + 2 * p], a1 = perm[off + 2 * p + 1];
-                next[off + p] = static_cast<uint16_t>((s ? a1 : a0) >> 1);
-                next[off + lc + p] = static_cast<uint16_t>((s ? a0 : a1) >> 1);
+                next[off + p] = static_cast<uint16_t>((sv ? a1 : a0) >> 1);
+                next[off + lc + p] = static_cast<uint16_t>((sv ? a0 : a1) >> 1);
             } else if ((c & 1) && p == h) {
                 next[off + h] = static_cast<uint16_t>(perm[off + c - 1] >> 1);
             }
