@@ -63,7 +63,6 @@ def algorithmic_bytes(kernel: str, n: int, blob: int) -> float:
     kernel covers one depth of the whole permutation)."""
     per_launch = {
         "k_hook_first": 6.0 * n,        # inverse read (4 B/entry), every slot's parent word written (4 B/slot)
-        "k_hook_second": 4.0 * n,       # inverse read
         "k_union": 4.0 * n,             # inverse read
         "k_route_inputs": 10.0 * n + n / 8,   # P read, children written, slot words read, row bits
         "k_route_outputs": 8.0 * n + n / 4,   # I read, children written, row bits

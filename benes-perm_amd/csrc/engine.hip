@@ -543,8 +543,6 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             if (lp.hbm.count) {
                 BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W>), grid(lp.hbm),
                           dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc, errs);
-                BH_LAUNCH(prof, "k_hook_second", (k_hook_second<VI, W>), grid(lp.hbm),
-                          dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc, errs);
                 BH_LAUNCH(prof, "k_union", (k_union<VI, W>), grid(lp.hbm), dim3(kBlock), 0, st,
                           segp(lp.hbm), lp.hbm.count, inv, uf, desc, errs);
                 BH_LAUNCH(prof, "k_route_inputs", (k_route_inputs<W>), grid(lp.hbm), dim3(kBlock),

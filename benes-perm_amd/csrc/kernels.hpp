@@ -17,7 +17,7 @@
 //   k_scatter_inverse  validation and the inverse permutation, no atomics
 // Per "global" level (nodes larger than one workgroup's LDS):
 //   k_describe       one thread per node: offsets and both sides' addresses
-//   k_hook_first, k_hook_second, k_union
+//   k_hook_first, k_union
 //                    one thread per output pair: parity union-find over the
 //                    node's input slots in HBM (one CAS per hook)
 //   k_route_inputs   one thread per input pair: switch values, routing of P,
@@ -281,7 +281,7 @@ __device__ __forceinline__ uint64_t in_slot(uint64_t x, uint64_t count) {
     return x >> 1;
 }
 
-// The colouring's union-find runs in three passes over the output pairs.
+// The colouring's union-find runs in two passes over the output pairs.
 // Every input slot has exactly one even entry (2p for pair p, the even tail
 // entry count - 1 for slot 0 of an odd node), and each entry reaches one
 // output pair, whose edge joins the entry's slot to the other entry's slot.
@@ -289,20 +289,16 @@ __device__ __forceinline__ uint64_t in_slot(uint64_t x, uint64_t count) {
 //                  the relation to the edge's other slot when that slot is
 //                  smaller, else the root word.  One plain store per slot, no
 //                  atomics; it replaces a fill of root words.
-//   k_hook_second  an edge reached by its larger slot's odd entry hooks that
-//                  slot under the smaller one when the first pass left it a
-//                  root (only this thread writes that word in this pass).
-//   k_union        the remaining edges: those reached by their larger slot's
-//                  odd entry whose larger slot was already hooked by the first
-//                  pass, i.e. slots larger than both neighbours (about a third
-//                  of the edges of a random permutation), joined by a Rem walk
-//                  with the compare-and-swap hook.
-// Edges reached by their larger slot's even entry were settled by the first
-// pass (the larger slot was hooked through that very edge).  Every stored
-// word is a true parity relation to a smaller slot, so the forest invariants
-// of the union-find hold across the passes; a pass-three edge whose larger
-// slot already holds the edge's own relation (hooked by the second pass, or
-// rewritten to it by halving) is skipped.
+//   k_union        the edges reached by their larger slot's odd entry (the
+//                  others were settled by the first pass, which hooked the
+//                  larger slot through that very edge).  A larger slot the
+//                  first pass left a root is hooked straight under the smaller
+//                  one with a compare-and-swap; otherwise (slots larger than
+//                  both neighbours, about a third of the edges of a random
+//                  permutation) a Rem walk joins the two trees.
+// Every stored word is a true parity relation to a smaller slot, so the
+// forest invariants of the union-find hold across the passes; an edge whose
+// larger slot already holds the edge's own relation is skipped.
 struct PairEdge {
     uint64_t lo, hi;    // the two input slots, lo < hi (equal: a 2-cycle)
     uint32_t label;     // value(lo) ^ value(hi)
@@ -354,27 +350,6 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
 }
 
 template <typename VI, typename W>
-__global__ __launch_bounds__(kBlock) void k_hook_second(const Seg* __restrict__ segs,
-                                                        uint32_t nsegs,
-                                                        const VI* __restrict__ inv,
-                                                        W* __restrict__ uf,
-                                                        const NodeDesc* __restrict__ desc,
-                                                        const int* __restrict__ errs) {
-    const Seg& sg = find_seg(segs, nsegs);
-    if (errs[sg.item]) return;
-    uint64_t node, chunk;
-    if (!block_to_node(sg, node, chunk, false)) return;
-    const NodeDesc& nd = desc[sg.desc0 + node];
-    const uint64_t off = nd.off, cnt = nd.cnt;
-    const uint64_t q = chunk * kBlock + threadIdx.x;
-    if (q >= (cnt >> 1)) return;
-    const PairEdge e = pair_edge(inv, off, cnt, q);
-    if (e.lo == e.hi || !e.hi_odd) return;
-    W* u = uf + off;
-    if (u[e.hi] == UfWord<W>::root) u[e.hi] = (static_cast<W>(e.lo) << 1) | e.label;
-}
-
-template <typename VI, typename W>
 __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, uint32_t nsegs,
                                                   const VI* __restrict__ inv, W* __restrict__ uf,
                                                   const NodeDesc* __restrict__ desc,
@@ -391,12 +366,13 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
     if (e.lo == e.hi || !e.hi_odd) return;
     W* u = uf + off;
     const W lo = static_cast<W>(e.lo), hi = static_cast<W>(e.hi);
-    const W w = DeviceForest::load(u + hi);
+    W w = DeviceForest::load(u + hi);
+    if (w == UfWord<W>::root) {
+        w = DeviceForest::hook(u + hi, (lo << 1) | e.label);
+        if (w == UfWord<W>::root) return;
+    }
     if (w == ((lo << 1) | e.label)) return;
-    // w is never the root word after the first two passes; the general walk
-    // below would handle it.
-    if (w == UfWord<W>::root) uf_union_rem<DeviceForest>(u, hi, 0u, lo, 0u, e.label);
-    else uf_union_rem<DeviceForest>(u, w >> 1, static_cast<uint32_t>(w & 1), lo, 0u, e.label);
+    uf_union_rem<DeviceForest>(u, w >> 1, static_cast<uint32_t>(w & 1), lo, 0u, e.label);
 }
 
 // Entry i (node-local index `off + i` in shared terms) of the level's

@@ -7,15 +7,17 @@
 // slots labelled 1 ^ (I[2q] & 1) ^ (I[2q + 1] & 1): the XOR of the two slots'
 // switch values (a slot's colour is that of its even entry).  A parent word
 // holds (parent slot << 1) | parity-to-parent; the all-ones word marks a
-// root.  Hooking always puts the larger root under the smaller one, so every
-// tree is rooted at its minimum slot, the tail being slot 0: each component's
-// root is the vertex the reference colours 0 first (benes.cpp:278-297: the
-// odd tail, else the component's smallest input), and a slot's parity to its
-// root is exactly its canonical switch value.
+// root.  A slot is only ever hooked under a smaller slot (a root under a
+// smaller root, or under any smaller vertex in the Rem walk), so parents are
+// below children and every tree is rooted at its minimum slot, the tail being
+// slot 0: each component's root is the vertex the reference colours 0 first
+// (benes.cpp:278-297: the odd tail, else the component's smallest input), and
+// a slot's parity to its root is exactly its canonical switch value.
 //
 // Why cached loads and plain stores are enough.  Every word ever stored is a
 // true parity relation from a slot to a strictly smaller slot: a hook links a
-// root to a smaller slot, and path halving links a non-root to one of its
+// root (or, in the first pass over the edges, a slot no other thread writes)
+// to a smaller slot, and path halving links a non-root to one of its
 // ancestors.  A slot that has a parent never becomes a root again, so a load
 // that returns an out-of-date word still returns a true relation to a smaller
 // slot.  Therefore
