@@ -36,6 +36,7 @@ EXPORTS = (
     "benes_hip_available",
     "benes_hip_compress",
     "benes_hip_compress_device",
+    "benes_hip_compress_file",
     "benes_hip_batch_layout",
     "benes_hip_compress_batch",
     "benes_hip_compress_batch_device",
@@ -177,6 +178,8 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_release_workspace.restype = None
     lib.benes_hip_autotune_report.argtypes = [cp, sz]
     lib.benes_hip_set_subtree_shape.argtypes = [u32, u32]
+    lib.benes_hip_compress_file.argtypes = [cp, cp, u32, fp, ctypes.POINTER(u64),
+                                            ctypes.POINTER(ctypes.c_double), cp, sz]
     lib.benes_hip_lookup.argtypes = [vp, u64, vp, u64, ctypes.c_int, vp, u32, cp, sz]
     lib.benes_hip_lookup_device.argtypes = [vp, u64, vp, u64, ctypes.c_int, vp, u32, vp, cp, sz]
     _lib = lib
@@ -243,6 +246,21 @@ def compress_into(perm: np.ndarray, fmt: FormatOptions | None = None,
                                 ctypes.byref(size), ctypes.byref(secs), err, 256)
     _raise(rc, err)
     return out[:size.value]
+
+
+def compress_file(input_path, output_path, width: int = 4,
+                  fmt: FormatOptions | None = None) -> int:
+    """The reference's compress_file: raw little-endian values of `width`
+    bytes from `input_path`, the blob written to `output_path` (streamed
+    through pinned memory both ways).  Returns the blob's size."""
+    f, keep = _format(fmt)
+    size = ctypes.c_uint64()
+    err = ctypes.create_string_buffer(256)
+    _raise(library().benes_hip_compress_file(os.fsencode(input_path), os.fsencode(output_path),
+                                             width, ctypes.byref(f), ctypes.byref(size), None,
+                                             err, 256), err)
+    del keep
+    return size.value
 
 
 def compress(perm: np.ndarray, fmt: FormatOptions | None = None) -> bytes:

@@ -23,9 +23,7 @@ namespace {
     }
 }
 
-template <typename Value>
-std::vector<std::byte> compress_hip(std::span<const Value> permutation,
-                                    const FormatOptions& format) {
+benes_hip_format c_format(const FormatOptions& format) {
     benes_hip_format f{};
     f.middle_group_log2 = format.middle_group_log2;
     f.middle_placement = static_cast<std::uint8_t>(format.middle_placement);
@@ -36,6 +34,20 @@ std::vector<std::byte> compress_hip(std::span<const Value> permutation,
     f.n_output_clusters = static_cast<std::uint8_t>(format.output_clusters.size());
     f.input_clusters = format.input_clusters.data();
     f.output_clusters = format.output_clusters.data();
+    return f;
+}
+
+void require_hip(Backend backend) {
+    if (backend == Backend::automatic) backend = Backend::hip;
+    if (backend != Backend::hip || !detail::hip_available()) {
+        throw std::runtime_error(std::string(backend_name(backend)) + " backend is unavailable");
+    }
+}
+
+template <typename Value>
+std::vector<std::byte> compress_hip(std::span<const Value> permutation,
+                                    const FormatOptions& format) {
+    const benes_hip_format f = c_format(format);
     char err[256] = {0};
     std::uint64_t size = 0;
     // Size query first: errors in the input size or format surface here.
@@ -54,11 +66,7 @@ std::vector<std::byte> compress_hip(std::span<const Value> permutation,
 
 template <typename Value>
 std::vector<std::byte> dispatch(std::span<const Value> permutation, CompressOptions options) {
-    Backend backend = options.backend;
-    if (backend == Backend::automatic) backend = Backend::hip;
-    if (backend != Backend::hip || !detail::hip_available()) {
-        throw std::runtime_error(std::string(backend_name(backend)) + " backend is unavailable");
-    }
+    require_hip(options.backend);
     return compress_hip(permutation, options.format);
 }
 
@@ -72,6 +80,17 @@ std::vector<std::byte> compress(std::span<const std::uint32_t> permutation,
 std::vector<std::byte> compress(std::span<const std::uint64_t> permutation,
                                 CompressOptions options) {
     return dispatch(permutation, std::move(options));
+}
+
+void compress_file(const std::filesystem::path& input, const std::filesystem::path& output,
+                   InputValueWidth input_width, CompressOptions options) {
+    require_hip(options.backend);
+    const benes_hip_format f = c_format(options.format);
+    char err[256] = {0};
+    const int rc = benes_hip_compress_file(input.c_str(), output.c_str(),
+                                           static_cast<std::uint32_t>(input_width), &f, nullptr,
+                                           nullptr, err, sizeof(err));
+    if (rc != BENES_HIP_OK) rethrow(rc, err);
 }
 
 bool backend_available(Backend backend) noexcept {

@@ -6,7 +6,8 @@
 // enumerator added, Backend::hip.  Only the compression path is provided here:
 // Backend::automatic resolves to hip, and any other backend reports itself
 // unavailable exactly as the reference does for a backend it was built
-// without (src/benes.cpp:437-442).  Lookup, views and files stay with the
+// without (src/benes.cpp:437-442).  compress_file is the reference's
+// (src/benes.cpp:498-553).  Lookup, views and file readers stay with the
 // reference library, which reads the byte-identical blobs this produces.
 //
 // detail::hip_available / detail::compress_compact_hip are the backend
@@ -17,6 +18,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <filesystem>
 #include <span>
 #include <vector>
 
@@ -61,6 +63,12 @@ struct CompressOptions {
                                               CompressOptions options = {});
 [[nodiscard]] std::vector<std::byte> compress(std::span<const std::uint64_t> permutation,
                                               CompressOptions options = {});
+
+// Raw little-endian values from `input`, the blob to `output` (streamed
+// through pinned memory to and from the GPU, with no host copy of either).
+void compress_file(const std::filesystem::path& input, const std::filesystem::path& output,
+                   InputValueWidth input_width = InputValueWidth::bits64,
+                   CompressOptions options = {});
 
 [[nodiscard]] bool backend_available(Backend backend) noexcept;
 [[nodiscard]] const char* backend_name(Backend backend) noexcept;

@@ -149,6 +149,26 @@ int benes_hip_server_stats(benes_hip_server* server, benes_hip_server_counters* 
 /* Finishes queued and running requests, then stops the loop. */
 void benes_hip_server_destroy(benes_hip_server* server);
 
+/* ---- Files: ingest and egress. ----
+ * Replaces compress_file (src/benes.cpp:498-553): raw little-endian values of
+ * width_bytes (4 or 8) from `input_path`, the blob written to `output_path`.
+ * The input is read chunk by chunk straight into pinned memory and on to HBM
+ * (validated there), and the blob streams from HBM into the output file; no
+ * host copy of either is made.  Errors and messages are the reference's, in
+ * its order: "cannot open permutation input file" (BENES_HIP_RUNTIME),
+ * "permutation input has an invalid element width or size"
+ * (BENES_HIP_INVALID_ARGUMENT), "failed to read permutation input file"
+ * (BENES_HIP_RUNTIME), the compress errors, then "failed to write compressed
+ * permutation file" (BENES_HIP_RUNTIME).  The output is created or truncated
+ * only after compression succeeded.  *out_size receives the blob's size.
+ * This build returns BENES_HIP_RUNTIME ("compress_file is not available in
+ * this build") without touching either file: the streaming path is switched
+ * off. */
+int benes_hip_compress_file(const char* input_path, const char* output_path,
+                            uint32_t width_bytes, const benes_hip_format* format,
+                            uint64_t* out_size, double* device_seconds, char* err,
+                            size_t err_len);
+
 /* ---- Lookup: the consumer side of the format. ----
  * Replaces the reference's forward_lookup / inverse_lookup
  * (src/internal.hpp:1036-1139), as used by CompressedPermutationView
