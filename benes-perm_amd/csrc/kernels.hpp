@@ -228,10 +228,11 @@ __global__ void k_headers(const ItemCtx* __restrict__ items, uint32_t count,
 // left P[p] = P[2p + s_p] >> 1 and left I[q] = I[2q + t_q] >> 1.
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
-// When an item has at least 8 nodes at a depth (its segment then starts on a
-// multiple of 8), blocks are renumbered so that every block of node j lands
-// on XCD j % 8, the node's union-find words, arrays and rows then living in
-// that one XCD's L2.  Blocks of one XCD then either take its nodes one after
+// HBM-level segments start on a multiple of 8 blocks.  When an item has at
+// least 8 nodes at a depth, blocks are renumbered so that every block of node
+// j lands on XCD j % 8, the node's union-find words, arrays and rows then
+// living in that one XCD's L2; with 2 or 4 nodes each node gets 4 or 2 XCDs
+// of its own.  Blocks of one XCD then either take its nodes one after
 // another (`spread` false: the blocks in flight share one node, whose words
 // stay in L2), or visit its nodes in turn (`spread` true: the blocks in flight
 // cover all of them).  The union pass spreads: its compare-and-swaps go past
@@ -242,8 +243,10 @@ __device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uin
                                               bool spread) {
     const uint64_t b = blockIdx.x - sg.block0;
     if (sg.nodes % kXcds != 0) {
-        node = b / sg.chunks;
-        chunk = b % sg.chunks;
+        // 1, 2 or 4 nodes (the segment starts on a multiple of 8): block b
+        // serves node b mod nodes, so each node keeps to its own XCDs.
+        node = b % sg.nodes;
+        chunk = b / sg.nodes;
     } else {
         const uint64_t xcd = b % kXcds, local = b / kXcds, per_xcd = sg.nodes / kXcds;
         if (spread) {
