@@ -235,9 +235,10 @@ __global__ void k_headers(const ItemCtx* __restrict__ items, uint32_t count,
 // of its own.  Blocks of one XCD then either take its nodes one after
 // another (`spread` false: the blocks in flight share one node, whose words
 // stay in L2), or visit its nodes in turn (`spread` true: the blocks in flight
-// cover all of them).  The union pass spreads: its compare-and-swaps go past
+// cover up to kSpreadNodes of them).  The union pass spreads: its compare-and-swaps go past
 // L2 to memory, and crowding them on one node's words serialises them there.
 constexpr uint32_t kXcds = 8;
+constexpr uint64_t kSpreadNodes = 64;
 
 __device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uint64_t& chunk,
                                               bool spread) {
@@ -250,8 +251,13 @@ __device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uin
     } else {
         const uint64_t xcd = b % kXcds, local = b / kXcds, per_xcd = sg.nodes / kXcds;
         if (spread) {
-            node = (local % per_xcd) * kXcds + xcd;
-            chunk = local / per_xcd;
+            // At most kSpreadNodes of the XCD's nodes at a time, so that very
+            // large items (thousands of nodes per XCD) keep the live words
+            // within reach of the address translation caches.
+            const uint64_t g = per_xcd < kSpreadNodes ? per_xcd : kSpreadNodes;
+            const uint64_t grp = local / (g * sg.chunks), in_grp = local - grp * (g * sg.chunks);
+            node = (grp * g + in_grp % g) * kXcds + xcd;
+            chunk = in_grp / g;
         } else {
             node = (local / sg.chunks) * kXcds + xcd;
             chunk = local % sg.chunks;
