@@ -314,11 +314,14 @@ struct PairEdge {
     bool hi_odd;        // the larger slot reaches this pair through its odd entry
 };
 
+// The level arrays are streamed once per kernel and read again only by the
+// next one; their loads and stores are non-temporal so that they do not push
+// the union-find words out of L2.
 template <typename VI>
 __device__ __forceinline__ PairEdge pair_edge(const VI* inv, uint64_t off, uint64_t cnt,
                                               uint64_t q) {
-    const uint64_t a = static_cast<uint64_t>(inv[off + 2 * q]);
-    const uint64_t b = static_cast<uint64_t>(inv[off + 2 * q + 1]);
+    const uint64_t a = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q));
+    const uint64_t b = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q + 1));
     const uint64_t sa = in_slot(a, cnt), sb = in_slot(b, cnt);
     PairEdge e;
     e.lo = sa < sb ? sa : sb;
@@ -344,8 +347,8 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
     const uint64_t q = chunk * kBlock + threadIdx.x;
     W* u = uf + off;
     if (q < h) {
-        const uint64_t a = static_cast<uint64_t>(inv[off + 2 * q]);
-        const uint64_t b = static_cast<uint64_t>(inv[off + 2 * q + 1]);
+        const uint64_t a = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q));
+        const uint64_t b = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q + 1));
         const uint64_t sa = in_slot(a, cnt), sb = in_slot(b, cnt);
         const W label = static_cast<W>(1 ^ ((a ^ b) & 1));
         if (!(a & 1)) u[sa] = sb < sa ? (static_cast<W>(sb) << 1) | label : UfWord<W>::root;
@@ -388,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
 // permutation: the item's own source at depth 0, the shared level array below.
 __device__ __forceinline__ uint64_t load_perm(const ItemCtx& it, const uint32_t* lev, bool level0,
                                               uint64_t abs) {
-    return level0 ? load_src(it, abs - it.base) : static_cast<uint64_t>(lev[abs]);
+    return level0 ? load_src(it, abs - it.base) : static_cast<uint64_t>(__builtin_nontemporal_load(lev + abs));
 }
 
 // One thread per input pair: canonical switch value s_p (its slot's parity to
@@ -423,8 +426,8 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restri
         if (r != x0) DeviceForest::store(u + x0, (r << 1) | static_cast<W>(s));
         const uint64_t a0 = load_perm(it, plev, level0, off + 2 * p);
         const uint64_t a1 = load_perm(it, plev, level0, off + 2 * p + 1);
-        pdst[off + p] = static_cast<uint32_t>((s ? a1 : a0) >> 1);
-        pdst[off + lc + p] = static_cast<uint32_t>((s ? a0 : a1) >> 1);
+        __builtin_nontemporal_store(static_cast<uint32_t>((s ? a1 : a0) >> 1), pdst + off + p);
+        __builtin_nontemporal_store(static_cast<uint32_t>((s ? a0 : a1) >> 1), pdst + off + lc + p);
     } else if ((cnt & 1) && p == h) {
         pdst[off + h] = static_cast<uint32_t>(load_perm(it, plev, level0, off + cnt - 1) >> 1);
     }
@@ -454,12 +457,12 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict_
     const uint64_t q = chunk * kBlock + threadIdx.x;
     uint32_t t = 0;
     if (q < h) {
-        const uint64_t a0 = static_cast<uint64_t>(inv[off + 2 * q]);
-        const uint64_t a1 = static_cast<uint64_t>(inv[off + 2 * q + 1]);
+        const uint64_t a0 = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q));
+        const uint64_t a1 = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q + 1));
         // colour(x) = s_{x>>1} ^ (x & 1); the odd tail entry has colour 0.
         if (!((cnt & 1) && a0 == cnt - 1)) t = row_bit(srows, row, a0 >> 1) ^ (a0 & 1);
-        idst[off + q] = static_cast<uint32_t>((t ? a1 : a0) >> 1);
-        idst[off + lc + q] = static_cast<uint32_t>((t ? a0 : a1) >> 1);
+        __builtin_nontemporal_store(static_cast<uint32_t>((t ? a1 : a0) >> 1), idst + off + q);
+        __builtin_nontemporal_store(static_cast<uint32_t>((t ? a0 : a1) >> 1), idst + off + lc + q);
     } else if ((cnt & 1) && q == h) {
         idst[off + h] = static_cast<uint32_t>(static_cast<uint64_t>(inv[off + cnt - 1]) >> 1);
     }
