@@ -474,7 +474,10 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict_
 // One workgroup per node of at most kLevelLds entries: the same three steps
 // as k_union / k_route_inputs / k_route_outputs, with the node's union-find
 // words and its input switch values in LDS instead of HBM.  A node of 2^16
-// entries has 2^15 input slots: 128 KiB of parent words.
+// entries has 2^15 input slots: 64 KiB of 16-bit parent words, so two
+// 1024-thread workgroups share a CU.  (16-bit words do not pay in k_subtree,
+// where the CU holds the same number of workgroups either way and the
+// emulated 16-bit compare-and-swap costs more than it saves.)
 constexpr uint32_t kLevelLdsLog2 = 16;
 constexpr uint32_t kLevelLds = 1u << kLevelLdsLog2;
 constexpr uint32_t kLevelThreads = 1024;
@@ -487,8 +490,11 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
     uint64_t* __restrict__ trows, const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
-    __shared__ uint32_t uf[kLevelLds / 2 + 1];
+    // 16-bit parent words (slots below 2^15), so two workgroups fit a CU.
+    static_assert(kLevelLds / 2 <= (1u << 15), "LDS-level slots must fit 16-bit words");
+    __shared__ uint32_t uf_words[kLevelLds / 4 + 1];
     __shared__ uint64_t sbits[kLevelLds / 128 + 1];
+    uint16_t* const uf = reinterpret_cast<uint16_t*>(uf_words);
     const ItemCtx it = items[sg.item];
     const uint64_t node = blockIdx.x - sg.block0;
     const NodeDesc& nd = desc[sg.desc0 + node];
@@ -497,13 +503,13 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
     const uint32_t h = cnt >> 1, odd = cnt & 1, lc = h + odd;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nthreads = blockDim.x, nwaves = blockDim.x >> 6;
-    for (uint32_t i = tid; i < lc; i += nthreads) uf[i] = UfWord<uint32_t>::root;
+    for (uint32_t i = tid; i < (lc + 1) / 2; i += nthreads) uf_words[i] = 0xFFFFFFFFu;
     __syncthreads();
     for (uint32_t q = tid; q < h; q += nthreads) {
         const uint32_t a = static_cast<uint32_t>(isrc[off + 2 * q]);
         const uint32_t b = static_cast<uint32_t>(isrc[off + 2 * q + 1]);
-        const uint32_t sa = static_cast<uint32_t>(in_slot(a, cnt));
-        const uint32_t sb = static_cast<uint32_t>(in_slot(b, cnt));
+        const uint16_t sa = static_cast<uint16_t>(in_slot(a, cnt));
+        const uint16_t sb = static_cast<uint16_t>(in_slot(b, cnt));
         if (sa != sb) uf_union_slots<GroupForest>(uf, sa, sb, 1u ^ ((a ^ b) & 1u));
     }
     __syncthreads();
@@ -512,7 +518,7 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
         const uint32_t p = base + lane;
         uint32_t s = 0;
         if (p < h) {
-            uf_find<GroupForest>(uf, p + odd, s);
+            uf_find<GroupForest>(uf, static_cast<uint16_t>(p + odd), s);
             const uint64_t a0 = load_perm(it, plev, level0, off + 2 * p);
             const uint64_t a1 = load_perm(it, plev, level0, off + 2 * p + 1);
             pdst[off + p] = static_cast<uint32_t>((s ? a1 : a0) >> 1);

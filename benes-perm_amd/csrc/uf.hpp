@@ -43,6 +43,12 @@ namespace benes_hip {
 
 template <typename W>
 struct UfWord;
+// 16-bit words hold slots below 2^15: a parent is at most the largest slot
+// minus one, so no parent word reaches the all-ones root mark.
+template <>
+struct UfWord<uint16_t> {
+    static constexpr uint16_t root = 0xFFFFu;
+};
 template <>
 struct UfWord<uint32_t> {
     static constexpr uint32_t root = 0xFFFFFFFFu;
@@ -89,6 +95,27 @@ struct GroupForest {
                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         return expected;
     }
+    // A 16-bit word shares an aligned 32-bit LDS word with its neighbour:
+    // the hook is a 32-bit compare-and-swap, retried while only the
+    // neighbour's half changed (a halving store or a hook of the neighbour).
+    static __device__ __forceinline__ uint16_t hook(uint16_t* p, uint16_t desired) {
+        const uint32_t hi_half = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(p) >> 1) & 1u;
+        uint32_t* const w = reinterpret_cast<uint32_t*>(p - hi_half);
+        const uint32_t sh = hi_half * 16;
+        uint32_t cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        while (true) {
+            const uint16_t mine = static_cast<uint16_t>(cur >> sh);
+            if (mine != UfWord<uint16_t>::root) return mine;
+            const uint32_t want = (cur & ~(0xFFFFu << sh)) | (static_cast<uint32_t>(desired) << sh);
+            uint32_t expected = cur;
+            if (__hip_atomic_compare_exchange_strong(w, &expected, want, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                return UfWord<uint16_t>::root;
+            }
+            cur = expected;
+        }
+    }
 };
 
 // Root of x and the parity of x relative to it (path halving on the way).
@@ -106,8 +133,8 @@ __device__ __forceinline__ W uf_find(W* uf, W x, uint32_t& parity) {
             x = p;
             break;
         }
-        const W hop = (w ^ w2) & 1;
-        F::store(uf + x, (w2 & ~W{1}) | hop);
+        const W hop = static_cast<W>((w ^ w2) & 1);
+        F::store(uf + x, static_cast<W>((w2 & ~W{1}) | hop));
         par ^= static_cast<uint32_t>(hop);
         x = w2 >> 1;
     }
@@ -125,11 +152,11 @@ __device__ __forceinline__ void uf_link(W* uf, W ru, uint32_t pu, W rv, uint32_t
         const bool u_high = ru > rv;
         const W hi = u_high ? ru : rv, lo = u_high ? rv : ru;
         const W rel = static_cast<W>((pu ^ pv ^ label) & 1);
-        const W found = F::hook(uf + hi, (lo << 1) | rel);
+        const W found = F::hook(uf + hi, static_cast<W>((lo << 1) | rel));
         if (found == kRoot) return;
         // hi already has a parent: continue from it.
         uint32_t q;
-        const W r = uf_find<F>(uf, found >> 1, q);
+        const W r = uf_find<F>(uf, static_cast<W>(found >> 1), q);
         q ^= static_cast<uint32_t>(found & 1);
         if (u_high) {
             ru = r;
@@ -149,11 +176,11 @@ __device__ __forceinline__ void uf_link(W* uf, W ru, uint32_t pu, W rv, uint32_t
 template <typename F, typename W>
 __device__ __forceinline__ void uf_union_slots(W* uf, W a, W b, uint32_t label) {
     const W hi = a > b ? a : b, lo = a > b ? b : a;
-    const W found = F::hook(uf + hi, (lo << 1) | static_cast<W>(label & 1));
+    const W found = F::hook(uf + hi, static_cast<W>((lo << 1) | (label & 1)));
     if (found == UfWord<W>::root) return;
     uint32_t pl, ph;
     const W rl = uf_find<F>(uf, lo, pl);
-    const W rh = uf_find<F>(uf, found >> 1, ph);
+    const W rh = uf_find<F>(uf, static_cast<W>(found >> 1), ph);
     uf_link<F>(uf, rl, pl, rh, ph ^ static_cast<uint32_t>(found & 1), label);
 }
 
