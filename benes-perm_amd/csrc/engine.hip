@@ -96,7 +96,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     DevBuf input, level[2], ilevel[2], inv0, uf, srows, trows, desc, plans, items, segs, errs,
-        blob;
+        blob, deferred, deferred_counts;
     std::vector<int> host_errs;
     PinnedRing ring;  // host <-> device staging of this workspace's stream
     Workspace() = default;
@@ -498,6 +498,8 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         ws.desc.ensure(sizeof(NodeDesc) * max_desc);
         ws.srows.ensure(8 * max_rows);
         ws.trows.ensure(8 * max_rows);
+        ws.deferred.ensure(sizeof(DeferredEdge) * kDeferCap);
+        ws.deferred_counts.ensure(sizeof(unsigned long long) * 2 * max_df);
     }
     BH_CHECK(hipMemcpyAsync(ws.plans.ptr, plans.data(), sizeof(Plan) * count,
                             hipMemcpyHostToDevice, st));
@@ -508,6 +510,10 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     BH_CHECK(hipMemsetAsync(ws.errs.ptr, 0, sizeof(int) * count, st));
     BH_CHECK(hipMemsetAsync(blob_ptr, 0, total_words * 8, st));
     BH_CHECK(hipMemsetAsync(ws.inv0.ptr, 0xFF, total_n * (wide_inverse ? 8 : 4), st));
+    if (max_df > 0) {
+        BH_CHECK(hipMemsetAsync(ws.deferred_counts.ptr, 0, sizeof(unsigned long long) * 2 * max_df,
+                                st));
+    }
 
     const ItemCtx* ditems = ws.items.as<ItemCtx>();
     const Seg* dsegs = ws.segs.as<Seg>();
@@ -547,8 +553,17 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             if (lp.hbm.count) {
                 BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W>), grid(lp.hbm),
                           dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc, errs);
+                // Per depth: edges deferred, flatten blocks finished.
+                unsigned long long* const dcount =
+                    ws.deferred_counts.as<unsigned long long>() + 2 * d;
+                DeferredEdge* const deferred = ws.deferred.as<DeferredEdge>();
                 BH_LAUNCH(prof, "k_union", (k_union<VI, W>), grid(lp.hbm), dim3(kBlock), 0, st,
-                          segp(lp.hbm), lp.hbm.count, inv, uf, desc, errs);
+                          segp(lp.hbm), lp.hbm.count, inv, uf, desc, deferred, dcount, errs);
+                // Returns at once unless k_union deferred an edge.
+                BH_LAUNCH(prof, "k_flatten", (k_flatten<W>),
+                          dim3(static_cast<uint32_t>(std::min<uint64_t>(lp.hbm.blocks, 2048))),
+                          dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, lp.hbm.blocks, uf, desc,
+                          deferred, dcount, errs);
                 BH_LAUNCH(prof, "k_route_inputs", (k_route_inputs<W>), grid(lp.hbm), dim3(kBlock),
                           0, st, ditems, segp(lp.hbm), lp.hbm.count, level0, plev, pdst, uf, desc,
                           srows, errs);

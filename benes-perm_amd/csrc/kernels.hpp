@@ -77,9 +77,8 @@ struct Seg {
     uint32_t pad;
 };
 
-// Segment of the block: the last one whose first block is <= blockIdx.x.
-__device__ __forceinline__ const Seg& find_seg(const Seg* segs, uint32_t nsegs) {
-    const uint64_t b = blockIdx.x;
+// Segment of block b: the last one whose first block is <= b.
+__device__ __forceinline__ const Seg& find_seg_at(const Seg* segs, uint32_t nsegs, uint64_t b) {
     uint32_t lo = 0, hi = nsegs - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
@@ -87,6 +86,10 @@ __device__ __forceinline__ const Seg& find_seg(const Seg* segs, uint32_t nsegs) 
         else hi = mid - 1;
     }
     return segs[lo];
+}
+
+__device__ __forceinline__ const Seg& find_seg(const Seg* segs, uint32_t nsegs) {
+    return find_seg_at(segs, nsegs, blockIdx.x);
 }
 
 // Entry i of an item's level-0 permutation.
@@ -240,9 +243,9 @@ __global__ void k_headers(const ItemCtx* __restrict__ items, uint32_t count,
 constexpr uint32_t kXcds = 8;
 constexpr uint64_t kSpreadNodes = 64;
 
-__device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uint64_t& chunk,
-                                              bool spread) {
-    const uint64_t b = blockIdx.x - sg.block0;
+__device__ __forceinline__ bool block_to_node_at(const Seg& sg, uint64_t block, uint64_t& node,
+                                                 uint64_t& chunk, bool spread) {
+    const uint64_t b = block - sg.block0;
     if (sg.nodes % kXcds != 0) {
         // 1, 2 or 4 nodes (the segment starts on a multiple of 8): block b
         // serves node b mod nodes, so each node keeps to its own XCDs.
@@ -264,6 +267,11 @@ __device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uin
         }
     }
     return node < sg.nodes && chunk < sg.chunks;
+}
+
+__device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uint64_t& chunk,
+                                              bool spread) {
+    return block_to_node_at(sg, blockIdx.x, node, chunk, spread);
 }
 
 __global__ void k_describe(const ItemCtx* __restrict__ items, const Seg* __restrict__ segs,
@@ -361,10 +369,27 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
     }
 }
 
+// Rem walks longer than this many steps are handed to k_flatten.  A walk
+// follows one dependent load per step; on random
+// permutations the longest walk of a level is a few dozen steps, but the
+// first-hook pass chains every slot under its neighbour for some structured
+// ones (a reverse of odd length, a rotation by one of even length), and the
+// edge closing such a cycle would walk half the node one load at a time.
+constexpr uint32_t kWalkCap = 1024;
+// Deferred edges per level; beyond this an edge walks its whole path itself.
+constexpr uint64_t kDeferCap = uint64_t{1} << 18;
+struct DeferredEdge {
+    uint64_t off;       // the node's first union-find word
+    uint64_t lo_label;  // smaller slot, label in bit 63
+    uint64_t hi;        // larger slot
+};
+
 template <typename VI, typename W>
 __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, uint32_t nsegs,
                                                   const VI* __restrict__ inv, W* __restrict__ uf,
                                                   const NodeDesc* __restrict__ desc,
+                                                  DeferredEdge* __restrict__ deferred,
+                                                  unsigned long long* __restrict__ deferred_count,
                                                   const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
@@ -384,7 +409,69 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
         if (w == UfWord<W>::root) return;
     }
     if (w == ((lo << 1) | e.label)) return;
-    uf_union_rem<DeviceForest>(u, w >> 1, static_cast<uint32_t>(w & 1), lo, 0u, e.label);
+    if (uf_union_rem<DeviceForest>(u, w >> 1, static_cast<uint32_t>(w & 1), lo, 0u, e.label,
+                                   kWalkCap)) {
+        return;
+    }
+    // A long path: flattened for the edge by k_flatten, unless the list is full.
+    const unsigned long long i = atomicAdd(deferred_count, 1ull);
+    if (i < kDeferCap) {
+        deferred[i] = DeferredEdge{nd.off, e.lo | (static_cast<uint64_t>(e.label) << 63), e.hi};
+        return;
+    }
+    uf_union_rem<DeviceForest>(u, w >> 1, static_cast<uint32_t>(w & 1), lo, 0u, e.label,
+                               ~uint64_t{0});
+}
+
+// Runs only when k_union deferred an edge at this depth (`counts[0]`):
+// every slot finds its root (halving, then pointing the slot at the root), as
+// k_route_inputs does.  Run by all slots at once this is pointer jumping, so
+// chains collapse in a logarithmic number of rounds instead of being walked
+// end to end.  The last block to finish (`counts[1]` counts finished blocks)
+// then joins the deferred edges, whose walks are now short.  The grid is
+// small and strides over the level's blocks, so one launch that reads a zero
+// count is all a level pays when nothing was deferred.
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_flatten(const Seg* __restrict__ segs, uint32_t nsegs,
+                                                    uint64_t vblocks, W* __restrict__ uf,
+                                                    const NodeDesc* __restrict__ desc,
+                                                    const DeferredEdge* __restrict__ deferred,
+                                                    unsigned long long* __restrict__ counts,
+                                                    const int* __restrict__ errs) {
+    const unsigned long long total = counts[0];
+    if (total == 0) return;
+    for (uint64_t b = blockIdx.x; b < vblocks; b += gridDim.x) {
+        const Seg& sg = find_seg_at(segs, nsegs, b);
+        if (errs[sg.item]) continue;
+        uint64_t node, chunk;
+        if (!block_to_node_at(sg, b, node, chunk, false)) continue;
+        const NodeDesc& nd = desc[sg.desc0 + node];
+        const uint64_t cnt = nd.cnt;
+        const uint64_t p = chunk * kBlock + threadIdx.x;
+        if (p >= (cnt >> 1)) continue;
+        W* const u = uf + nd.off;
+        const W x0 = static_cast<W>(p + (cnt & 1));
+        uint32_t s;
+        const W r = uf_find<DeviceForest>(u, x0, s);
+        if (r != x0) DeviceForest::store(u + x0, static_cast<W>((r << 1) | static_cast<W>(s)));
+    }
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(&counts[1], 1ull) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const uint64_t n = total < kDeferCap ? total : kDeferCap;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const DeferredEdge e = deferred[i];
+        const W lo = static_cast<W>(e.lo_label & ~(uint64_t{1} << 63));
+        const uint32_t label = static_cast<uint32_t>(e.lo_label >> 63);
+        uf_union_rem<DeviceForest>(uf + e.off, static_cast<W>(e.hi), 0u, lo, 0u, label,
+                                   ~uint64_t{0});
+    }
 }
 
 // Entry i (node-local index `off + i` in shared terms) of the level's

@@ -194,11 +194,15 @@ __device__ __forceinline__ void uf_union_slots(W* uf, W a, W b, uint32_t label) 
 // root of a large tree, so the words near big roots are read far less.  The
 // walk writes nothing but the hook: halving here costs more stores than the
 // later finds save.
+//
+// At most `max_steps` parent words are read; the walk returns false if it
+// stops there, having written nothing, and true once the relation is recorded.
 template <typename F, typename W>
-__device__ __forceinline__ void uf_union_rem(W* uf, W x, uint32_t px, W y, uint32_t py,
-                                             uint32_t label) {
+__device__ __forceinline__ bool uf_union_rem(W* uf, W x, uint32_t px, W y, uint32_t py,
+                                             uint32_t label, uint64_t max_steps) {
     constexpr W kRoot = UfWord<W>::root;
-    while (x != y) {
+    for (uint64_t step = 0; x != y; ++step) {
+        if (step == max_steps) return false;
         if (x < y) {
             const W t = x;
             x = y;
@@ -210,12 +214,13 @@ __device__ __forceinline__ void uf_union_rem(W* uf, W x, uint32_t px, W y, uint3
         W w = F::load(uf + x);
         if (w == kRoot) {
             const W found = F::hook(uf + x, (y << 1) | static_cast<W>((px ^ py ^ label) & 1));
-            if (found == kRoot) return;
+            if (found == kRoot) return true;
             w = found;
         }
         px ^= static_cast<uint32_t>(w & 1);
         x = w >> 1;
     }
+    return true;
 }
 
 // Slot numbering inside a node of `count` entries (h = count/2 pairs).

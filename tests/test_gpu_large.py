@@ -121,8 +121,6 @@ def test_generators_agree_with_host():
         assert np.array_equal(d.cpu().numpy().view(np.uint64), B.generate(kind, n, 99, 0, 8))
 
 
-@pytest.mark.skip(reason="a run of this test stopped making progress (no output for 180 s) "
-                         "and its cause has not been found")
 def test_reverse_closed_form(big):
     blob = run(big, B.REVERSE)
     lay = big[3]
