@@ -139,6 +139,41 @@ def other_inputs(d_out, size, stream, steps: int):
     return out
 
 
+def structured_inputs(stream, steps: int):
+    """Structured 2^26-scale inputs whose level graphs are long chains for the
+    union-find (a rotation by one, an odd-length reverse and perfect shuffle),
+    each byte-checked against the reference library's digest."""
+    import support as S
+
+    with open(os.path.join(ROOT, "tests", "golden", "structured.json")) as f:
+        records = [r for r in json.load(f) if r["n"] >= N]
+    out = {}
+    for rec in records:
+        n = rec["n"]
+        fmt = B.FormatOptions(rec["g"], tuple(rec["ic"]), tuple(rec["oc"]), rec["placement"])
+        size = B.encoded_size(n, fmt)
+        perm = S.structured(rec["kind"], n).astype(np.uint32)
+        d_perm = torch.from_numpy(perm.view(np.int32)).cuda()
+        del perm
+        d_out = torch.empty(size, dtype=torch.uint8, device="cuda")
+
+        def step():
+            B.compress_device(d_perm.data_ptr(), n, 4, d_out.data_ptr(), size, fmt, stream)
+
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        digest = hashlib.sha256(d_out.cpu().numpy().tobytes()).hexdigest()
+        out[rec["name"]] = {"ms_per_step": dt * 1e3, "Mentries_per_s": n / dt / 1e6,
+                            "byte_exact": digest == rec["sha256"]}
+        del d_perm, d_out
+    return out
+
+
 def device_lookup(d_out, size: int, d_perm, reps: int = 3):
     """Whole decode of the 2^26 blob on the device (the reference's
     forward_lookup for every entry), checked against the input."""
@@ -504,6 +539,7 @@ def main():
     if not args.no_extra:
         line["lookup_2e26"] = device_lookup(d_out, size, d_perm)
         line["inputs_2e26"] = other_inputs(d_out, size, stream, max(3, args.steps // 2))
+        line["structured_2e26"] = structured_inputs(stream, 3)
         line["host_inclusive"] = host_inclusive(perm, want)
         line["batched"] = batched(args.batch_items)
         line["served"] = served(args.requests, args.offered_rps)

@@ -6,6 +6,8 @@ seeded inputs from benes_hip.generate().
 
 Run in a container where the reference is mounted and built:
     make -C oracle && make -C benes-perm_amd && python tests/golden/make_fixtures.py
+(`--structured` rewrites only tests/golden/structured.json, the digests of
+the structured inputs of support.structured).
 
 Each record holds the input recipe (generator kind, n, seed, rate, width) and
 the format, so the same input can be rebuilt anywhere without the reference.
@@ -62,6 +64,36 @@ def cases():
     return out
 
 
+def structured_cases():
+    """Structured inputs (support.structured) at the 2^26 scale, whose level
+    graphs are long chains for the union-find."""
+    small = [dict(name=f"{kind}-{n}", kind=kind, n=n, width=4, g=5, placement=pl, ic=[], oc=[])
+             for kind, n, pl in (("reverse", 65537, 0), ("rotate1", 65536, 1), ("rotate3", 65536, 2),
+                                 ("rotate_half", 131072, 0), ("shuffle", 65537, 1),
+                                 ("reverse_halves", 131073, 2))]
+    return small + [
+        dict(name="rotate1-2^26", kind="rotate1", n=1 << 26, width=4, **CFG2_FMT),
+        dict(name="reverse-2^26+1", kind="reverse", n=(1 << 26) + 1, width=4, g=5, placement=1,
+             ic=[], oc=[]),
+        dict(name="shuffle-2^26+1", kind="shuffle", n=(1 << 26) + 1, width=4, g=5, placement=1,
+             ic=[], oc=[]),
+    ]
+
+
+def write_structured():
+    records = []
+    for c in structured_cases():
+        perm = S.structured(c["kind"], c["n"])
+        blob = S.ref_compress(perm, c["g"], c["placement"], c["ic"], c["oc"], threads=8)
+        rec = dict(c)
+        rec["size"] = len(blob)
+        rec["sha256"] = hashlib.sha256(blob).hexdigest()
+        records.append(rec)
+        print(rec["name"], rec["size"], rec["sha256"][:16], flush=True)
+    with open(os.path.join(HERE, "structured.json"), "w") as f:
+        json.dump(records, f, indent=1)
+
+
 def main():
     if not S.ref_available():
         raise SystemExit("oracle/_ref/libbenes_ref.so is not built (needs the reference sources)")
@@ -77,7 +109,13 @@ def main():
         print(rec["name"], rec["size"], rec["sha256"][:16], flush=True)
     with open(os.path.join(HERE, "fixtures.json"), "w") as f:
         json.dump(records, f, indent=1)
+    write_structured()
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--structured"]:
+        if not S.ref_available():
+            raise SystemExit("oracle/_ref/libbenes_ref.so is not built (needs the reference sources)")
+        write_structured()
+    else:
+        main()

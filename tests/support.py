@@ -200,6 +200,27 @@ def oracle_address(layout: OrcLayout, side: int, depth: int, path: int) -> OrcAd
     return out
 
 
+def structured(kind: str, n: int) -> np.ndarray:
+    """Structured permutations of 0..n-1 (uint64) whose level graphs are long
+    monotone chains for the union-find: a reverse, rotations by 1, 3 and
+    n/2 + 1, a perfect shuffle (evens then odds) and a reversal of each half."""
+    i = np.arange(n, dtype=np.uint64)
+    if kind == "reverse":
+        return i[::-1].copy()
+    if kind == "rotate1":
+        return (i + 1) % n
+    if kind == "rotate3":
+        return (i + 3) % n
+    if kind == "rotate_half":
+        return (i + n // 2 + 1) % n
+    if kind == "shuffle":
+        h = (n + 1) // 2
+        return np.where(i < h, 2 * i, 2 * (i - h) + 1).astype(np.uint64)
+    if kind == "reverse_halves":
+        return np.concatenate([i[: n // 2][::-1], i[n // 2:][::-1]])
+    raise ValueError(f"unknown structured permutation {kind!r}")
+
+
 def fmt(g=5, placement=0, ic=(), oc=()):
     return benes_hip.FormatOptions(g, tuple(ic), tuple(oc), placement)
 

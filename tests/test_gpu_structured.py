@@ -14,6 +14,9 @@ end-to-end walk of 2^23 loads costs.
 """
 from __future__ import annotations
 
+import hashlib
+import json
+import os
 import time
 
 import numpy as np
@@ -30,31 +33,13 @@ def device():
     assert B.available(), "no gfx950 device or libbenes_hip.so failed to load its kernels"
 
 
-def structured(kind: str, n: int) -> np.ndarray:
-    i = np.arange(n, dtype=np.uint64)
-    if kind == "reverse":
-        return i[::-1].copy()
-    if kind == "rotate1":
-        return (i + 1) % n
-    if kind == "rotate3":
-        return (i + 3) % n
-    if kind == "rotate_half":
-        return (i + n // 2 + 1) % n
-    if kind == "shuffle":
-        h = (n + 1) // 2
-        return np.where(i < h, 2 * i, 2 * (i - h) + 1).astype(np.uint64)
-    if kind == "reverse_halves":
-        return np.concatenate([i[: n // 2][::-1], i[n // 2:][::-1]])
-    raise ValueError(kind)
-
-
 KINDS = ["reverse", "rotate1", "rotate3", "rotate_half", "shuffle", "reverse_halves"]
 
 
 @pytest.mark.parametrize("n", [(1 << 20) + 1, 1 << 21])
 @pytest.mark.parametrize("kind", KINDS)
 def test_structured_matches_oracle(kind, n):
-    perm = structured(kind, n)
+    perm = S.structured(kind, n)
     assert np.array_equal(np.sort(perm), np.arange(n, dtype=np.uint64))
     print(f"structured {kind} n={n}", flush=True)
     got = B.compress(perm.astype(np.uint32), S.fmt(5, 1))
@@ -63,14 +48,14 @@ def test_structured_matches_oracle(kind, n):
 
 def test_structured_wide_values():
     n = (1 << 21) + 3
-    perm = structured("reverse", n)
+    perm = S.structured("reverse", n)
     got = B.compress(perm, S.fmt(4, 2))
     assert got == S.oracle_compress(perm, 4, 2)
 
 
 def test_rotation_2e24_is_fast():
     n = 1 << 24
-    perm = structured("rotate1", n).astype(np.uint32)
+    perm = S.structured("rotate1", n).astype(np.uint32)
     fmt = S.fmt(5, 0)
     B.compress(perm, fmt)
     t0 = time.perf_counter()
@@ -81,3 +66,18 @@ def test_rotation_2e24_is_fast():
     idx = np.random.default_rng(24).integers(0, n, 4096, dtype=np.uint64)
     assert np.array_equal(B.lookup(blob, idx), (idx + 1) % n)
     assert np.array_equal(B.lookup(blob, idx, inverse=True), (idx + n - 1) % n)
+
+
+def structured_records():
+    with open(os.path.join(S.GOLDEN, "structured.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("rec", structured_records(), ids=lambda r: r["name"])
+def test_structured_reference_digests(rec):
+    # Digests of the reference library's own blobs at 2^26 scale.
+    perm = S.structured(rec["kind"], rec["n"]).astype(np.uint32 if rec["width"] == 4 else np.uint64)
+    print(f"structured digest {rec['name']}", flush=True)
+    blob = B.compress(perm, S.fmt(rec["g"], rec["placement"], rec["ic"], rec["oc"]))
+    assert len(blob) == rec["size"]
+    assert hashlib.sha256(blob).hexdigest() == rec["sha256"]

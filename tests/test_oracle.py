@@ -249,3 +249,18 @@ def test_oracle_matches_reference_digests(rec):
     blob = S.oracle_compress(perm, rec["g"], rec["placement"], rec["ic"], rec["oc"])
     assert len(blob) == rec["size"]
     assert hashlib.sha256(blob).hexdigest() == rec["sha256"]
+
+
+def load_structured():
+    with open(os.path.join(S.GOLDEN, "structured.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("rec", [r for r in load_structured() if r["n"] <= (1 << 17) + 1],
+                         ids=lambda r: r["name"])
+def test_oracle_matches_reference_structured(rec):
+    # Inputs whose level graphs are long chains (support.structured).
+    perm = S.structured(rec["kind"], rec["n"])
+    blob = S.oracle_compress(perm, rec["g"], rec["placement"], rec["ic"], rec["oc"])
+    assert len(blob) == rec["size"]
+    assert hashlib.sha256(blob).hexdigest() == rec["sha256"]
