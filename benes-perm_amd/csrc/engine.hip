@@ -425,6 +425,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     struct LevelPlan {
         Launch describe, hbm, lds, pack[2];
         bool wide_uf = false;
+        uint64_t hbm_entries = 0;  // entries of the items coloured in HBM at this depth
     };
     std::vector<LevelPlan> levels(max_df);
     uint64_t max_desc = 1, max_rows = 1;
@@ -443,6 +444,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             if (cmax > kLevelLds) {
                 const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
                 hbm.add(i, nodes * chunks, nodes, chunks, desc_count, row_count, row_words, true);
+                lp.hbm_entries += jobs[i].n;
                 lp.wide_uf = lp.wide_uf || maxpairs + 1 >= (uint64_t{1} << 31);
             } else {
                 lds.add(i, nodes, nodes, 1, desc_count, row_count, row_words, false);
@@ -499,7 +501,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         ws.srows.ensure(8 * max_rows);
         ws.trows.ensure(8 * max_rows);
         ws.deferred.ensure(sizeof(DeferredEdge) * kDeferCap);
-        ws.deferred_counts.ensure(sizeof(unsigned long long) * 2 * max_df);
+        ws.deferred_counts.ensure(sizeof(unsigned long long) * 4 * max_df);
     }
     BH_CHECK(hipMemcpyAsync(ws.plans.ptr, plans.data(), sizeof(Plan) * count,
                             hipMemcpyHostToDevice, st));
@@ -511,7 +513,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     BH_CHECK(hipMemsetAsync(blob_ptr, 0, total_words * 8, st));
     BH_CHECK(hipMemsetAsync(ws.inv0.ptr, 0xFF, total_n * (wide_inverse ? 8 : 4), st));
     if (max_df > 0) {
-        BH_CHECK(hipMemsetAsync(ws.deferred_counts.ptr, 0, sizeof(unsigned long long) * 2 * max_df,
+        BH_CHECK(hipMemsetAsync(ws.deferred_counts.ptr, 0, sizeof(unsigned long long) * 4 * max_df,
                                 st));
     }
 
@@ -551,22 +553,43 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             using VI = std::remove_const_t<std::remove_pointer_t<decltype(inv)>>;
             using W = std::remove_pointer_t<decltype(uf)>;
             if (lp.hbm.count) {
-                BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W>), grid(lp.hbm),
-                          dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc, errs);
-                // Per depth: edges deferred, flatten blocks finished.
+                // Per depth: edges deferred, flatten blocks finished, hooks
+                // under the slot just below.
                 unsigned long long* const dcount =
-                    ws.deferred_counts.as<unsigned long long>() + 2 * d;
+                    ws.deferred_counts.as<unsigned long long>() + 4 * d;
+                // Fewer than 8 nodes: each node's blocks spread over several
+                // XCDs.  Coherent walks pay only there, and only when chains
+                // of neighbouring slots make up more than 1/64 of the slots
+                // (counted in every eighth block).
+                const bool spread = d < 3;
+                const unsigned long long chain_threshold =
+                    spread ? lp.hbm_entries / 2 / 64 / 8 : ~0ull;
+                if (spread) {
+                    BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W, true>), grid(lp.hbm),
+                              dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
+                              dcount + 2, chain_threshold, errs);
+                } else {
+                    BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W, false>), grid(lp.hbm),
+                              dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
+                              dcount + 2, chain_threshold, errs);
+                }
                 DeferredEdge* const deferred = ws.deferred.as<DeferredEdge>();
                 BH_LAUNCH(prof, "k_union", (k_union<VI, W>), grid(lp.hbm), dim3(kBlock), 0, st,
                           segp(lp.hbm), lp.hbm.count, inv, uf, desc, deferred, dcount, errs);
                 // Returns at once unless k_union deferred an edge.
-                BH_LAUNCH(prof, "k_flatten", (k_flatten<W>),
-                          dim3(static_cast<uint32_t>(std::min<uint64_t>(lp.hbm.blocks, 2048))),
-                          dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, lp.hbm.blocks, uf, desc,
-                          deferred, dcount, errs);
+                const dim3 flat_grid(static_cast<uint32_t>(std::min<uint64_t>(lp.hbm.blocks, 2048)));
+                if (spread) {
+                    BH_LAUNCH(prof, "k_flatten", (k_flatten<W, CoherentForest>), flat_grid,
+                              dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, lp.hbm.blocks, uf,
+                              desc, deferred, dcount, errs);
+                } else {
+                    BH_LAUNCH(prof, "k_flatten", (k_flatten<W, DeviceForest>), flat_grid,
+                              dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, lp.hbm.blocks, uf,
+                              desc, deferred, dcount, errs);
+                }
                 BH_LAUNCH(prof, "k_route_inputs", (k_route_inputs<W>), grid(lp.hbm), dim3(kBlock),
                           0, st, ditems, segp(lp.hbm), lp.hbm.count, level0, plev, pdst, uf, desc,
-                          srows, errs);
+                          srows, dcount + 2, chain_threshold, errs);
                 BH_LAUNCH(prof, "k_route_outputs", (k_route_outputs<VI>), grid(lp.hbm),
                           dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, idst, desc, srows,
                           trows, errs);

@@ -339,12 +339,20 @@ __device__ __forceinline__ PairEdge pair_edge(const VI* inv, uint64_t off, uint6
     return e;
 }
 
-template <typename VI, typename W>
+// With kCount (depths whose nodes spread over several XCDs), every eighth
+// block counts its slots hooked under the slot just below them (`adjacent`),
+// up to just past `chain_threshold` (a wave adds its count only while the
+// total is at most the threshold, so the one counter sees few atomics): many
+// of them mean chains of neighbouring slots (rotations, reversals), which
+// k_route_inputs then walks with loads and stores coherent across XCDs.
+template <typename VI, typename W, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ segs,
                                                        uint32_t nsegs,
                                                        const VI* __restrict__ inv,
                                                        W* __restrict__ uf,
                                                        const NodeDesc* __restrict__ desc,
+                                                       unsigned long long* __restrict__ adjacent,
+                                                       unsigned long long chain_threshold,
                                                        const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
@@ -354,6 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
     const uint64_t off = nd.off, cnt = nd.cnt, h = cnt >> 1;
     const uint64_t q = chunk * kBlock + threadIdx.x;
     W* u = uf + off;
+    bool next_to = false;
     if (q < h) {
         const uint64_t a = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q));
         const uint64_t b = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q + 1));
@@ -361,11 +370,20 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
         const W label = static_cast<W>(1 ^ ((a ^ b) & 1));
         if (!(a & 1)) u[sa] = sb < sa ? (static_cast<W>(sb) << 1) | label : UfWord<W>::root;
         if (!(b & 1)) u[sb] = sa < sb ? (static_cast<W>(sa) << 1) | label : UfWord<W>::root;
+        if (kCount && (blockIdx.x & 7) == 0) {
+            next_to = (!(a & 1) && sb + 1 == sa) || (!(b & 1) && sa + 1 == sb);
+        }
     } else if ((cnt & 1) && q == h) {
         // The unpaired output entry: no edge; an even entry there leaves its
         // slot a root.
         const uint64_t x = static_cast<uint64_t>(inv[off + cnt - 1]);
         if (!(x & 1)) u[in_slot(x, cnt)] = UfWord<W>::root;
+    }
+    if (!kCount || (blockIdx.x & 7) != 0) return;
+    const uint64_t ballot = __ballot(next_to);
+    if (ballot && lane_id() == 0 &&
+        __hip_atomic_load(adjacent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= chain_threshold) {
+        atomicAdd(adjacent, static_cast<unsigned long long>(__popcll(ballot)));
     }
 }
 
@@ -427,11 +445,13 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
 // every slot finds its root (halving, then pointing the slot at the root), as
 // k_route_inputs does.  Run by all slots at once this is pointer jumping, so
 // chains collapse in a logarithmic number of rounds instead of being walked
-// end to end.  The last block to finish (`counts[1]` counts finished blocks)
+// end to end, provided each XCD sees the others' shortcuts: F is
+// CoherentForest at depths whose nodes spread over several XCDs.  The last
+// block to finish (`counts[1]` counts finished blocks)
 // then joins the deferred edges, whose walks are now short.  The grid is
 // small and strides over the level's blocks, so one launch that reads a zero
 // count is all a level pays when nothing was deferred.
-template <typename W>
+template <typename W, typename F>
 __global__ __launch_bounds__(kBlock) void k_flatten(const Seg* __restrict__ segs, uint32_t nsegs,
                                                     uint64_t vblocks, W* __restrict__ uf,
                                                     const NodeDesc* __restrict__ desc,
@@ -452,8 +472,8 @@ __global__ __launch_bounds__(kBlock) void k_flatten(const Seg* __restrict__ segs
         W* const u = uf + nd.off;
         const W x0 = static_cast<W>(p + (cnt & 1));
         uint32_t s;
-        const W r = uf_find<DeviceForest>(u, x0, s);
-        if (r != x0) DeviceForest::store(u + x0, static_cast<W>((r << 1) | static_cast<W>(s)));
+        const W r = uf_find<F>(u, x0, s);
+        if (r != x0) F::store(u + x0, static_cast<W>((r << 1) | static_cast<W>(s)));
     }
     __shared__ bool last;
     __syncthreads();
@@ -469,8 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_flatten(const Seg* __restrict__ segs
         const DeferredEdge e = deferred[i];
         const W lo = static_cast<W>(e.lo_label & ~(uint64_t{1} << 63));
         const uint32_t label = static_cast<uint32_t>(e.lo_label >> 63);
-        uf_union_rem<DeviceForest>(uf + e.off, static_cast<W>(e.hi), 0u, lo, 0u, label,
-                                   ~uint64_t{0});
+        uf_union_rem<F>(uf + e.off, static_cast<W>(e.hi), 0u, lo, 0u, label, ~uint64_t{0});
     }
 }
 
@@ -483,6 +502,10 @@ __device__ __forceinline__ uint64_t load_perm(const ItemCtx& it, const uint32_t*
 
 // One thread per input pair: canonical switch value s_p (its slot's parity to
 // the root), child routing of P, and s ballot-packed into the node's row.
+// When the first-hook pass counted more than `chain_threshold` hooks under
+// the slot just below (chains of neighbouring slots) at a depth whose nodes
+// spread over several XCDs, the walks read and write coherently across XCDs,
+// so that each XCD's walks take the shortcuts the others wrote.
 template <typename W>
 __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restrict__ items,
                                                          const Seg* __restrict__ segs,
@@ -492,6 +515,8 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restri
                                                          W* __restrict__ uf,
                                                          const NodeDesc* __restrict__ desc,
                                                          uint64_t* __restrict__ srows,
+                                                         const unsigned long long* __restrict__ adjacent,
+                                                         unsigned long long chain_threshold,
                                                          const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
@@ -509,8 +534,13 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restri
         // parents are smaller, so later walks soon meet such words.
         W* const u = uf + off;
         const W x0 = static_cast<W>(p + (cnt & 1));
-        const W r = uf_find<DeviceForest>(u, x0, s);
-        if (r != x0) DeviceForest::store(u + x0, (r << 1) | static_cast<W>(s));
+        if (*adjacent > chain_threshold) {
+            const W r = uf_find<CoherentForest>(u, x0, s);
+            if (r != x0) CoherentForest::store(u + x0, (r << 1) | static_cast<W>(s));
+        } else {
+            const W r = uf_find<DeviceForest>(u, x0, s);
+            if (r != x0) DeviceForest::store(u + x0, (r << 1) | static_cast<W>(s));
+        }
         const uint64_t a0 = load_perm(it, plev, level0, off + 2 * p);
         const uint64_t a1 = load_perm(it, plev, level0, off + 2 * p + 1);
         __builtin_nontemporal_store(static_cast<uint32_t>((s ? a1 : a0) >> 1), pdst + off + p);

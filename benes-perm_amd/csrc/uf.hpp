@@ -78,6 +78,26 @@ struct DeviceForest {
     }
 };
 
+// The same arrays read and written coherently across XCDs: loads and stores
+// at agent scope bypass the XCD's own L2 copy (the L2s of different XCDs are
+// not coherent within a kernel).  Used where halving walks of several XCDs
+// cross the same long paths, which only pay off if each XCD sees the others'
+// shortcuts.
+struct CoherentForest {
+    template <typename W>
+    static __device__ __forceinline__ W load(W* p) {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    template <typename W>
+    static __device__ __forceinline__ void store(W* p, W v) {
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    template <typename W>
+    static __device__ __forceinline__ W hook(W* p, W desired) {
+        return DeviceForest::hook(p, desired);
+    }
+};
+
 // Parent arrays in LDS, private to one workgroup.
 struct GroupForest {
     template <typename W>
