@@ -423,7 +423,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     }
     const Launch per_entry = per_entry_list.commit(table);
     struct LevelPlan {
-        Launch describe, hbm, lds, pack[2];
+        Launch describe, hbm, lds, pack[2], narrow[2];
         bool wide_uf = false;
         uint64_t hbm_entries = 0;  // entries of the items coloured in HBM at this depth
     };
@@ -431,7 +431,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     uint64_t max_desc = 1, max_rows = 1;
     for (uint32_t d = 0; d < max_df; ++d) {
         LevelPlan& lp = levels[d];
-        SegList describe, hbm, lds, pack[2];
+        SegList describe, hbm, lds, pack[2], narrow[2];
         uint64_t desc_count = 0, row_count = 0;
         const uint64_t nodes = uint64_t{1} << d;
         for (uint32_t i = 0; i < count; ++i) {
@@ -459,6 +459,13 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                                ceil_div(groups * packets * pack_words_per_job(sd.group_depth),
                                         kBlock),
                                nodes, packets, desc_count, row_count, row_words, false);
+                if (pack_narrow(sd.group_depth, span_log2)) {
+                    // One block per (group, kNarrowGroups node groups, row word).
+                    const uint64_t node_groups =
+                        (uint64_t{1} << sd.relative_level) >> (6 - span_log2);
+                    narrow[side].add(i, groups * ceil_div(node_groups, kNarrowGroups) * row_words,
+                                     nodes, 0, desc_count, row_count, row_words, false);
+                }
             }
             desc_count += nodes;
             row_count += nodes * row_words;
@@ -468,6 +475,8 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         lp.lds = lds.commit(table);
         lp.pack[0] = pack[0].commit(table);
         lp.pack[1] = pack[1].commit(table);
+        lp.narrow[0] = narrow[0].commit(table);
+        lp.narrow[1] = narrow[1].commit(table);
         max_desc = std::max(max_desc, desc_count);
         max_rows = std::max(max_rows, row_count);
     }
@@ -480,7 +489,9 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     uint64_t most = subtree.blocks * sub.threads;
     for (const LevelPlan& lp : levels) {
         most = std::max({most, lp.hbm.blocks * kBlock, lp.lds.blocks * kLevelThreads,
-                         lp.pack[0].blocks * kBlock, lp.pack[1].blocks * kBlock});
+                         lp.pack[0].blocks * kBlock, lp.pack[1].blocks * kBlock,
+                         lp.narrow[0].blocks * kNarrowThreads,
+                         lp.narrow[1].blocks * kNarrowThreads});
     }
     if (most >= (uint64_t{1} << 32)) {
         throw std::length_error("batch too large for one call: split it into smaller batches");
@@ -617,6 +628,12 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             BH_LAUNCH(prof, "k_pack", k_pack, grid(lp.pack[side]), dim3(kBlock), 0, st, ditems,
                       segp(lp.pack[side]), lp.pack[side].count, side, d, desc,
                       side == 0 ? srows : trows, blob, errs);
+            if (lp.narrow[side].count) {
+                BH_LAUNCH(prof, "k_pack_narrow", k_pack_narrow, grid(lp.narrow[side]),
+                          dim3(kNarrowThreads), 0, st, ditems, segp(lp.narrow[side]),
+                          lp.narrow[side].count, side, d, desc, side == 0 ? srows : trows, blob,
+                          errs);
+            }
         }
     }
     // An item with fusion depth df was last written at depth df - 1, into

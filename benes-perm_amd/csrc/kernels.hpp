@@ -671,6 +671,15 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
     }
 }
 
+// Levels where a packet gives each node at most 4 bits (span <= 4) in runs
+// of at least 64 bits (group depth >= 7): their full-share packets are
+// packed by k_pack_narrow from whole row words, k_pack keeps the rest.  (At
+// wider spans a word gathers from few rows, and k_pack's per-word gather is
+// the cheaper of the two.)
+__host__ __device__ __forceinline__ bool pack_narrow(uint32_t group_depth, uint32_t span_log2) {
+    return span_log2 <= 2 && group_depth >= 7;
+}
+
 // Words of the blob one packet's share of a level (2^(g-1) bits, g the
 // cluster group's depth) can touch.
 __host__ __device__ __forceinline__ uint64_t pack_words_per_job(uint32_t g) {
@@ -717,6 +726,7 @@ __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ ite
     const uint64_t nodes = uint64_t{1} << r;
     const uint64_t secbase = (side == 0 ? P.input_offset : P.output_offset) * 8;
     if ((C >> r) > P.leaf && min_pairs >= first_pair + span) {
+        if (pack_narrow(g, span_log2)) return;  // k_pack_narrow's share
         const uint64_t len = nodes << span_log2;
         const uint64_t dst0 = secbase + bit_of(dsc[b0].a[side], first_pair);
         const uint64_t ow = (dst0 >> 6) + wi, wbit = ow << 6;
@@ -745,6 +755,75 @@ __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ ite
         const NodeDesc& nd = dsc[b0 + pi];
         lane_or_row(blob, secbase + bit_of(nd.a[side], first_pair), rows, nd.row, first_pair,
                     packet_share(nd.cnt, packet, span_log2, P.leaf));
+    }
+}
+
+// Full-share packets of the narrow levels (pack_narrow).  A packet's run is
+// the concatenation, over the group's nodes in path order, of each node's
+// span bits [first_pair, first_pair + span): the run's word k holds the
+// 64/span nodes of node group k.  A block takes (set, kNarrowGroups node
+// groups, one row word = 64 pairs), one wave per node group: lane L loads
+// the row word of the group's node L / span, and one ballot per packet of
+// those 64 pairs, over the bits (packet * span + L % span), is that
+// packet's word for the group.  One load per lane gives 64/span words, where
+// gathering each word from the rows costs 64/span loads of its own.  The
+// words go through LDS so that consecutive threads store consecutive words
+// of one run.
+constexpr uint32_t kNarrowGroups = 16;
+constexpr uint32_t kNarrowThreads = 64 * kNarrowGroups;
+
+__global__ __launch_bounds__(kNarrowThreads) void k_pack_narrow(
+    const ItemCtx* __restrict__ items, const Seg* __restrict__ segs, uint32_t nsegs, int side,
+    uint32_t d, const NodeDesc* __restrict__ desc, const uint64_t* __restrict__ rows,
+    uint64_t* __restrict__ blob_all, const int* __restrict__ errs) {
+    __shared__ uint64_t tile[64][kNarrowGroups];
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    const ItemCtx it = items[sg.item];
+    const Plan& P = *it.plan;
+    uint64_t* blob = blob_all + it.blob;
+    const SideDepth sd = P.side[side][d];
+    const uint32_t r = sd.relative_level, g = sd.group_depth;
+    const uint32_t span_log2 = g - r - 1;
+    const uint32_t span = 1u << span_log2;
+    const uint32_t per_word = 64u >> span_log2;  // nodes of a group; packets of a row word
+    const uint64_t node_groups = (uint64_t{1} << r) >> (6 - span_log2);
+    const uint64_t chunks = (node_groups + kNarrowGroups - 1) / kNarrowGroups;
+    const uint64_t rw = sg.row_words;
+    const uint64_t b = blockIdx.x - sg.block0;
+    const uint32_t root_depth = d - r;
+    const uint64_t set = b / (chunks * rw);
+    if (set >= (uint64_t{1} << root_depth)) return;
+    const uint64_t rest = b - set * (chunks * rw);
+    const uint64_t chunk = rest / rw, w = rest - chunk * rw;
+    const uint64_t C = node_count(P.n, root_depth, set);
+    if ((C >> r) <= P.leaf) return;
+    const uint64_t min_pairs = (C >> r) >> 1;
+    if ((w << 6) >= min_pairs) return;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t b0 = set << r;
+    const uint64_t ng = chunk * kNarrowGroups + wave;
+    if (ng < node_groups) {
+        const uint64_t node = b0 + ng * per_word + (lane >> span_log2);
+        const uint64_t word = rows[sg.row0 + node * rw + w];
+        for (uint32_t pk = 0; pk < per_word; ++pk) {
+            const uint64_t bits = __ballot(((word >> (pk * span + (lane & (span - 1)))) & 1) != 0);
+            if (lane == pk) tile[pk][wave] = bits;
+        }
+    }
+    __syncthreads();
+    const uint32_t pk = threadIdx.x / kNarrowGroups, gl = threadIdx.x % kNarrowGroups;
+    const uint64_t ngs = chunk * kNarrowGroups + gl;
+    if (pk >= per_word || ngs >= node_groups) return;
+    const uint64_t first_pair = ((w * per_word) + pk) << span_log2;
+    if (first_pair + span > min_pairs) return;  // not a full share: k_pack places it
+    const uint64_t secbase = (side == 0 ? P.input_offset : P.output_offset) * 8;
+    const uint64_t dst = secbase + bit_of(desc[sg.desc0 + b0].a[side], first_pair) + ngs * 64;
+    const uint64_t v = tile[pk][gl];
+    if ((dst & 63) == 0) {
+        blob[dst >> 6] = v;  // inside the run: no other writer
+    } else {
+        or_word(blob, dst, v);
     }
 }
 
