@@ -14,18 +14,28 @@
 // compress is a batch of one.
 //
 // Before the first level:
-//   k_scatter_inverse  validation and the inverse permutation, no atomics
-// Per "global" level (nodes larger than one workgroup's LDS):
+//   k_scatter_inverse, k_check_inverse
+//                    validation and the inverse permutation, no atomics
+// Per level, for every depth above the fusion depth:
 //   k_describe       one thread per node: offsets and both sides' addresses
+// Nodes larger than one workgroup's LDS (HBM levels):
 //   k_hook_first, k_union
 //                    one thread per output pair: parity union-find over the
-//                    node's input slots in HBM (one CAS per hook)
+//                    node's input slots in HBM (plain-store first hooks, then
+//                    one CAS per remaining hook; long walks deferred)
+//   k_flatten        pointer-jumping flatten and the deferred edges, only
+//                    when k_union deferred one
 //   k_route_inputs   one thread per input pair: switch values, routing of P,
 //                    values ballot-packed into node-major bit rows
 //   k_route_outputs  one thread per output pair: output switch values from
 //                    the input rows, routing of the inverse
-//   k_pack           one wave per (cluster group, packet): gathers a packet's
+// Nodes of at most 2^16 entries (LDS levels):
+//   k_level_lds      one workgroup per node: the same three steps in LDS
+// Both kinds of level, per side:
+//   k_pack           threads per (cluster group, packet): gather a packet's
 //                    run of switch bits from the rows into whole 64-bit words
+//   k_pack_narrow    the packets giving each node at most 4 bits, by ballots
+//                    over whole row words
 // Below the fusion depth:
 //   k_subtree   one workgroup per subtree: every remaining level in LDS, then
 //               the factoradic ranks of its leaves.
