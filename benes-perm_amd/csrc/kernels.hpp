@@ -621,6 +621,12 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
     static_assert(kLevelLds / 2 <= (1u << 15), "LDS-level slots must fit 16-bit words");
     __shared__ uint32_t uf_words[kLevelLds / 4 + 1];
     __shared__ uint64_t sbits[kLevelLds / 128 + 1];
+    // Unions whose finds met a long path (chains of neighbouring slots from
+    // structured permutations): (lo, hi, label) packed, joined after the
+    // workgroup has flattened the forest.
+    constexpr uint32_t kLdsDeferred = 64, kLdsWalkCap = 256;
+    __shared__ uint32_t deferred[kLdsDeferred];
+    __shared__ uint32_t ndeferred;
     uint16_t* const uf = reinterpret_cast<uint16_t*>(uf_words);
     const ItemCtx it = items[sg.item];
     const uint64_t node = blockIdx.x - sg.block0;
@@ -631,15 +637,42 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nthreads = blockDim.x, nwaves = blockDim.x >> 6;
     for (uint32_t i = tid; i < (lc + 1) / 2; i += nthreads) uf_words[i] = 0xFFFFFFFFu;
+    if (tid == 0) ndeferred = 0;
     __syncthreads();
     for (uint32_t q = tid; q < h; q += nthreads) {
         const uint32_t a = static_cast<uint32_t>(isrc[off + 2 * q]);
         const uint32_t b = static_cast<uint32_t>(isrc[off + 2 * q + 1]);
         const uint16_t sa = static_cast<uint16_t>(in_slot(a, cnt));
         const uint16_t sb = static_cast<uint16_t>(in_slot(b, cnt));
-        if (sa != sb) uf_union_slots<GroupForest>(uf, sa, sb, 1u ^ ((a ^ b) & 1u));
+        if (sa == sb) continue;
+        const uint32_t label = 1u ^ ((a ^ b) & 1u);
+        if (uf_union_slots_bounded<GroupForest>(uf, sa, sb, label, kLdsWalkCap)) continue;
+        const uint32_t i = atomicAdd(&ndeferred, 1u);
+        if (i < kLdsDeferred) {
+            deferred[i] = (static_cast<uint32_t>(sa) << 16) | (static_cast<uint32_t>(sb) << 1) | label;
+        } else {
+            uf_union_slots<GroupForest>(uf, sa, sb, label);
+        }
     }
     __syncthreads();
+    if (ndeferred != 0) {
+        // Every slot finds its root at once (pointer jumping), then the
+        // deferred unions walk short paths.
+        for (uint32_t p = tid; p < h; p += nthreads) {
+            uint32_t s;
+            const uint16_t x0 = static_cast<uint16_t>(p + odd);
+            const uint16_t r = uf_find<GroupForest>(uf, x0, s);
+            if (r != x0) GroupForest::store(uf + x0, static_cast<uint16_t>((r << 1) | s));
+        }
+        __syncthreads();
+        const uint32_t n = ndeferred < kLdsDeferred ? ndeferred : kLdsDeferred;
+        for (uint32_t i = tid; i < n; i += nthreads) {
+            const uint32_t e = deferred[i];
+            uf_union_slots<GroupForest>(uf, static_cast<uint16_t>(e >> 16),
+                                        static_cast<uint16_t>((e >> 1) & 0x7FFFu), e & 1u);
+        }
+        __syncthreads();
+    }
     // Input switch values: route P, keep s in LDS and in the node's row.
     for (uint32_t base = wave * 64; base < lc; base += nwaves * 64) {
         const uint32_t p = base + lane;

@@ -162,6 +162,34 @@ __device__ __forceinline__ W uf_find(W* uf, W x, uint32_t& parity) {
     return x;
 }
 
+// uf_find that reads at most 2 * max_steps parent words: false when it stops
+// short of the root (the halving it did stays; x is then partway up).
+template <typename F, typename W>
+__device__ __forceinline__ bool uf_find_bounded(W* uf, W& x, uint32_t& parity,
+                                                uint32_t max_steps) {
+    constexpr W kRoot = UfWord<W>::root;
+    uint32_t par = 0;
+    for (uint32_t step = 0; step < max_steps; ++step) {
+        const W w = F::load(uf + x);
+        if (w == kRoot) {
+            parity = par;
+            return true;
+        }
+        const W p = w >> 1;
+        const W w2 = F::load(uf + p);
+        if (w2 == kRoot) {
+            parity = par ^ static_cast<uint32_t>(w & 1);
+            x = p;
+            return true;
+        }
+        const W hop = static_cast<W>((w ^ w2) & 1);
+        F::store(uf + x, static_cast<W>((w2 & ~W{1}) | hop));
+        par ^= static_cast<uint32_t>(hop);
+        x = w2 >> 1;
+    }
+    return false;
+}
+
 // Records value(u) ^ value(v) == label, given u's root (ru, parity pu) and
 // v's root (rv, parity pv).
 template <typename F, typename W>
@@ -202,6 +230,23 @@ __device__ __forceinline__ void uf_union_slots(W* uf, W a, W b, uint32_t label) 
     const W rl = uf_find<F>(uf, lo, pl);
     const W rh = uf_find<F>(uf, static_cast<W>(found >> 1), ph);
     uf_link<F>(uf, rl, pl, rh, ph ^ static_cast<uint32_t>(found & 1), label);
+}
+
+// uf_union_slots whose two root finds give up after max_steps halving steps
+// each: false then, with nothing linked (the edge is to be joined later).
+template <typename F, typename W>
+__device__ __forceinline__ bool uf_union_slots_bounded(W* uf, W a, W b, uint32_t label,
+                                                       uint32_t max_steps) {
+    const W hi = a > b ? a : b, lo = a > b ? b : a;
+    const W found = F::hook(uf + hi, static_cast<W>((lo << 1) | (label & 1)));
+    if (found == UfWord<W>::root) return true;
+    uint32_t pl, ph;
+    W rl = lo, rh = static_cast<W>(found >> 1);
+    if (!uf_find_bounded<F>(uf, rl, pl, max_steps) || !uf_find_bounded<F>(uf, rh, ph, max_steps)) {
+        return false;
+    }
+    uf_link<F>(uf, rl, pl, rh, ph ^ static_cast<uint32_t>(found & 1), label);
+    return true;
 }
 
 // Union in the manner of Rem's algorithm: records value(x0) ^ value(y0) ==
