@@ -341,7 +341,8 @@ struct SegList {
     std::vector<Seg> segs;
     uint64_t blocks = 0;
     void add(uint32_t item, uint64_t blocks_of_item, uint64_t nodes, uint64_t chunks,
-             uint64_t desc0, uint64_t row0, uint64_t row_words, bool xcd_align) {
+             uint64_t desc0, uint64_t row0, uint64_t row_words, bool xcd_align,
+             uint32_t depth = 0) {
         if (blocks_of_item == 0) return;
         if (xcd_align) {
             blocks = round8(blocks);
@@ -356,6 +357,7 @@ struct SegList {
         s.row0 = row0;
         s.row_words = row_words;
         s.item = item;
+        s.depth = depth;
         segs.push_back(s);
         blocks += blocks_of_item;
     }
@@ -423,16 +425,19 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     }
     const Launch per_entry = per_entry_list.commit(table);
     struct LevelPlan {
-        Launch describe, hbm, lds, pack[2], narrow[2];
+        Launch hbm, lds, pack[2], narrow[2];
         bool wide_uf = false;
         uint64_t hbm_entries = 0;  // entries of the items coloured in HBM at this depth
     };
     std::vector<LevelPlan> levels(max_df);
-    uint64_t max_desc = 1, max_rows = 1;
+    // Descriptors of all depths live side by side (depth d's from desc_base),
+    // so one k_describe launch writes them all; rows are per depth.
+    SegList describe;
+    uint64_t desc_base = 0, max_rows = 1;
     for (uint32_t d = 0; d < max_df; ++d) {
         LevelPlan& lp = levels[d];
-        SegList describe, hbm, lds, pack[2], narrow[2];
-        uint64_t desc_count = 0, row_count = 0;
+        SegList hbm, lds, pack[2], narrow[2];
+        uint64_t desc_count = desc_base, row_count = 0;
         const uint64_t nodes = uint64_t{1} << d;
         for (uint32_t i = 0; i < count; ++i) {
             if (jobs[i].df <= d) continue;
@@ -440,7 +445,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             const uint64_t maxpairs = cmax / 2;
             const uint64_t row_words = std::max<uint64_t>(1, ceil_div(maxpairs, 64));
             describe.add(i, ceil_div(nodes, 64), nodes, 0, desc_count, row_count, row_words,
-                         false);
+                         false, d);
             if (cmax > kLevelLds) {
                 const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
                 hbm.add(i, nodes * chunks, nodes, chunks, desc_count, row_count, row_words, true);
@@ -470,23 +475,23 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             desc_count += nodes;
             row_count += nodes * row_words;
         }
-        lp.describe = describe.commit(table);
         lp.hbm = hbm.commit(table);
         lp.lds = lds.commit(table);
         lp.pack[0] = pack[0].commit(table);
         lp.pack[1] = pack[1].commit(table);
         lp.narrow[0] = narrow[0].commit(table);
         lp.narrow[1] = narrow[1].commit(table);
-        max_desc = std::max(max_desc, desc_count);
+        desc_base = desc_count;
         max_rows = std::max(max_rows, row_count);
     }
+    const Launch describe_all = describe.commit(table);
     SegList subtree_list;
     for (uint32_t i = 0; i < count; ++i) {
         subtree_list.add(i, uint64_t{1} << jobs[i].df, 0, 0, 0, 0, 0, false);
     }
     const Launch subtree = subtree_list.commit(table);
     // One dispatch holds at most 2^32 - 1 work-items.
-    uint64_t most = subtree.blocks * sub.threads;
+    uint64_t most = std::max(subtree.blocks * sub.threads, describe_all.blocks * 64);
     for (const LevelPlan& lp : levels) {
         most = std::max({most, lp.hbm.blocks * kBlock, lp.lds.blocks * kLevelThreads,
                          lp.pack[0].blocks * kBlock, lp.pack[1].blocks * kBlock,
@@ -508,7 +513,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         bool any_wide = false;
         for (const LevelPlan& lp : levels) any_wide = any_wide || lp.wide_uf;
         ws.uf.ensure(total_n * (any_wide ? 8 : 4));
-        ws.desc.ensure(sizeof(NodeDesc) * max_desc);
+        ws.desc.ensure(sizeof(NodeDesc) * std::max<uint64_t>(1, desc_base));
         ws.srows.ensure(8 * max_rows);
         ws.trows.ensure(8 * max_rows);
         ws.deferred.ensure(sizeof(DeferredEdge) * kDeferCap);
@@ -546,6 +551,10 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                        blob_ptr);
     if (wide_inverse) validate(ws.inv0.as<uint64_t>());
     else validate(ws.inv0.as<uint32_t>());
+    if (describe_all.count) {
+        BH_LAUNCH(prof, "k_describe", k_describe, grid(describe_all), dim3(64), 0, st, ditems,
+                  segp(describe_all), describe_all.count, ws.desc.as<NodeDesc>(), errs);
+    }
 
     // Depth d reads level[(d - 1) & 1] and writes level[d & 1].
     for (uint32_t d = 0; d < max_df; ++d) {
@@ -558,8 +567,6 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         uint32_t* pdst = ws.level[next].as<uint32_t>();
         uint32_t* idst = ws.ilevel[next].as<uint32_t>();
         const int level0 = d == 0 ? 1 : 0;
-        BH_LAUNCH(prof, "k_describe", k_describe, grid(lp.describe), dim3(64), 0, st, ditems,
-                  segp(lp.describe), lp.describe.count, d, desc, errs);
         auto run = [&](auto* inv, auto* uf) {
             using VI = std::remove_const_t<std::remove_pointer_t<decltype(inv)>>;
             using W = std::remove_pointer_t<decltype(uf)>;

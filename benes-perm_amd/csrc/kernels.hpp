@@ -84,7 +84,7 @@ struct Seg {
     uint64_t row0;       // first row word of the item at this depth
     uint64_t row_words;  // row words per node
     uint32_t item;
-    uint32_t pad;
+    uint32_t depth;      // the depth a k_describe segment describes
 };
 
 // Segment of block b: the last one whose first block is <= b.
@@ -284,11 +284,14 @@ __device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uin
     return block_to_node_at(sg, blockIdx.x, node, chunk, spread);
 }
 
+// Node descriptors of every depth above the fusion depth, in one launch:
+// each segment is one item at one depth (`depth`), its descriptors at desc0.
 __global__ void k_describe(const ItemCtx* __restrict__ items, const Seg* __restrict__ segs,
-                           uint32_t nsegs, uint32_t d, NodeDesc* __restrict__ desc,
+                           uint32_t nsegs, NodeDesc* __restrict__ desc,
                            const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
+    const uint32_t d = sg.depth;
     const ItemCtx it = items[sg.item];
     const Plan& P = *it.plan;
     const uint64_t j = (blockIdx.x - sg.block0) * blockDim.x + threadIdx.x;
