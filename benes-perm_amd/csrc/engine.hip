@@ -342,7 +342,7 @@ struct SegList {
     uint64_t blocks = 0;
     void add(uint32_t item, uint64_t blocks_of_item, uint64_t nodes, uint64_t chunks,
              uint64_t desc0, uint64_t row0, uint64_t row_words, bool xcd_align,
-             uint32_t depth = 0) {
+             uint32_t depth = 0, uint32_t side = 0) {
         if (blocks_of_item == 0) return;
         if (xcd_align) {
             blocks = round8(blocks);
@@ -358,6 +358,7 @@ struct SegList {
         s.row_words = row_words;
         s.item = item;
         s.depth = depth;
+        s.side = side;
         segs.push_back(s);
         blocks += blocks_of_item;
     }
@@ -425,7 +426,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     }
     const Launch per_entry = per_entry_list.commit(table);
     struct LevelPlan {
-        Launch hbm, lds, pack[2], narrow[2];
+        Launch hbm, lds, pack, narrow;  // pack, narrow: both sides' segments
         bool wide_uf = false;
         uint64_t hbm_entries = 0;  // entries of the items coloured in HBM at this depth
     };
@@ -436,7 +437,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     uint64_t desc_base = 0, max_rows = 1;
     for (uint32_t d = 0; d < max_df; ++d) {
         LevelPlan& lp = levels[d];
-        SegList hbm, lds, pack[2], narrow[2];
+        SegList hbm, lds, pack, narrow;
         uint64_t desc_count = desc_base, row_count = 0;
         const uint64_t nodes = uint64_t{1} << d;
         for (uint32_t i = 0; i < count; ++i) {
@@ -460,16 +461,15 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                 const uint32_t span_log2 = sd.group_depth - sd.relative_level - 1;
                 const uint64_t packets = ceil_div(maxpairs, uint64_t{1} << span_log2);
                 const uint64_t groups = uint64_t{1} << (d - sd.relative_level);
-                pack[side].add(i,
-                               ceil_div(groups * packets * pack_words_per_job(sd.group_depth),
-                                        kBlock),
-                               nodes, packets, desc_count, row_count, row_words, false);
+                pack.add(i,
+                         ceil_div(groups * packets * pack_words_per_job(sd.group_depth), kBlock),
+                         nodes, packets, desc_count, row_count, row_words, false, d, side);
                 if (pack_narrow(sd.group_depth, span_log2)) {
                     // One block per (group, kNarrowGroups node groups, row word).
                     const uint64_t node_groups =
                         (uint64_t{1} << sd.relative_level) >> (6 - span_log2);
-                    narrow[side].add(i, groups * ceil_div(node_groups, kNarrowGroups) * row_words,
-                                     nodes, 0, desc_count, row_count, row_words, false);
+                    narrow.add(i, groups * ceil_div(node_groups, kNarrowGroups) * row_words,
+                               nodes, 0, desc_count, row_count, row_words, false, d, side);
                 }
             }
             desc_count += nodes;
@@ -477,10 +477,8 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         }
         lp.hbm = hbm.commit(table);
         lp.lds = lds.commit(table);
-        lp.pack[0] = pack[0].commit(table);
-        lp.pack[1] = pack[1].commit(table);
-        lp.narrow[0] = narrow[0].commit(table);
-        lp.narrow[1] = narrow[1].commit(table);
+        lp.pack = pack.commit(table);
+        lp.narrow = narrow.commit(table);
         desc_base = desc_count;
         max_rows = std::max(max_rows, row_count);
     }
@@ -494,9 +492,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     uint64_t most = std::max(subtree.blocks * sub.threads, describe_all.blocks * 64);
     for (const LevelPlan& lp : levels) {
         most = std::max({most, lp.hbm.blocks * kBlock, lp.lds.blocks * kLevelThreads,
-                         lp.pack[0].blocks * kBlock, lp.pack[1].blocks * kBlock,
-                         lp.narrow[0].blocks * kNarrowThreads,
-                         lp.narrow[1].blocks * kNarrowThreads});
+                         lp.pack.blocks * kBlock, lp.narrow.blocks * kNarrowThreads});
     }
     if (most >= (uint64_t{1} << 32)) {
         throw std::length_error("batch too large for one call: split it into smaller batches");
@@ -631,16 +627,15 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             if (lp.wide_uf) run(isrc, ws.uf.as<uint64_t>());
             else run(isrc, ws.uf.as<uint32_t>());
         }
-        for (int side = 0; side < 2; ++side) {
-            BH_LAUNCH(prof, "k_pack", k_pack, grid(lp.pack[side]), dim3(kBlock), 0, st, ditems,
-                      segp(lp.pack[side]), lp.pack[side].count, side, d, desc,
-                      side == 0 ? srows : trows, blob, errs);
-            if (lp.narrow[side].count) {
-                BH_LAUNCH(prof, "k_pack_narrow", k_pack_narrow, grid(lp.narrow[side]),
-                          dim3(kNarrowThreads), 0, st, ditems, segp(lp.narrow[side]),
-                          lp.narrow[side].count, side, d, desc, side == 0 ? srows : trows, blob,
-                          errs);
-            }
+        // Both sides in one launch each (segments carry their side).
+        if (lp.pack.count) {
+            BH_LAUNCH(prof, "k_pack", k_pack, grid(lp.pack), dim3(kBlock), 0, st, ditems,
+                      segp(lp.pack), lp.pack.count, d, desc, srows, trows, blob, errs);
+        }
+        if (lp.narrow.count) {
+            BH_LAUNCH(prof, "k_pack_narrow", k_pack_narrow, grid(lp.narrow), dim3(kNarrowThreads),
+                      0, st, ditems, segp(lp.narrow), lp.narrow.count, d, desc, srows, trows,
+                      blob, errs);
         }
     }
     // An item with fusion depth df was last written at depth df - 1, into

@@ -85,6 +85,8 @@ struct Seg {
     uint64_t row_words;  // row words per node
     uint32_t item;
     uint32_t depth;      // the depth a k_describe segment describes
+    uint32_t side;       // the side a k_pack / k_pack_narrow segment packs
+    uint32_t pad;
 };
 
 // Segment of block b: the last one whose first block is <= b.
@@ -742,13 +744,15 @@ __host__ __device__ __forceinline__ uint64_t pack_words_per_job(uint32_t g) {
 // own address.
 __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ items,
                                                  const Seg* __restrict__ segs, uint32_t nsegs,
-                                                 int side, uint32_t d,
-                                                 const NodeDesc* __restrict__ desc,
-                                                 const uint64_t* __restrict__ rows,
+                                                 uint32_t d, const NodeDesc* __restrict__ desc,
+                                                 const uint64_t* __restrict__ srows,
+                                                 const uint64_t* __restrict__ trows,
                                                  uint64_t* __restrict__ blob_all,
                                                  const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
+    const int side = static_cast<int>(sg.side);
+    const uint64_t* const rows = side == 0 ? srows : trows;
     const ItemCtx it = items[sg.item];
     const Plan& P = *it.plan;
     uint64_t* blob = blob_all + it.blob;
@@ -819,12 +823,15 @@ constexpr uint32_t kNarrowGroups = 16;
 constexpr uint32_t kNarrowThreads = 64 * kNarrowGroups;
 
 __global__ __launch_bounds__(kNarrowThreads) void k_pack_narrow(
-    const ItemCtx* __restrict__ items, const Seg* __restrict__ segs, uint32_t nsegs, int side,
-    uint32_t d, const NodeDesc* __restrict__ desc, const uint64_t* __restrict__ rows,
-    uint64_t* __restrict__ blob_all, const int* __restrict__ errs) {
+    const ItemCtx* __restrict__ items, const Seg* __restrict__ segs, uint32_t nsegs, uint32_t d,
+    const NodeDesc* __restrict__ desc, const uint64_t* __restrict__ srows,
+    const uint64_t* __restrict__ trows, uint64_t* __restrict__ blob_all,
+    const int* __restrict__ errs) {
     __shared__ uint64_t tile[64][kNarrowGroups];
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
+    const int side = static_cast<int>(sg.side);
+    const uint64_t* const rows = side == 0 ? srows : trows;
     const ItemCtx it = items[sg.item];
     const Plan& P = *it.plan;
     uint64_t* blob = blob_all + it.blob;
