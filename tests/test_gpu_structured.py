@@ -81,3 +81,15 @@ def test_structured_reference_digests(rec):
     blob = B.compress(perm, S.fmt(rec["g"], rec["placement"], rec["ic"], rec["oc"]))
     assert len(blob) == rec["size"]
     assert hashlib.sha256(blob).hexdigest() == rec["sha256"]
+
+
+def test_structured_items_in_one_batch():
+    # Chain-structured and random items side by side: walk caps, deferred
+    # edges and the chain count are per depth over the whole batch.
+    perms = [S.structured("rotate1", 1 << 20), S.structured("reverse", (1 << 20) + 1),
+             np.random.default_rng(5).permutation(1 << 20).astype(np.uint64),
+             S.structured("shuffle", (1 << 19) + 1), S.structured("rotate1", (1 << 21) + 1)]
+    fmts = [S.fmt(5, 0), S.fmt(5, 1), S.fmt(4, 2), S.fmt(5, 2), S.fmt(3, 1)]
+    out = B.compress_batch([p.astype(np.uint32) for p in perms], fmts)
+    for p, f, blob in zip(perms, fmts, out):
+        assert blob == S.oracle_compress(p, f.middle_group_log2, f.middle_placement)
