@@ -429,6 +429,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         Launch hbm, lds, pack, narrow;  // pack, narrow: both sides' segments
         bool wide_uf = false;
         uint64_t hbm_entries = 0;  // entries of the items coloured in HBM at this depth
+        uint64_t max_slots = 0;    // input slots of the largest HBM node
     };
     std::vector<LevelPlan> levels(max_df);
     // Descriptors of all depths live side by side (depth d's from desc_base),
@@ -451,6 +452,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                 const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
                 hbm.add(i, nodes * chunks, nodes, chunks, desc_count, row_count, row_words, true);
                 lp.hbm_entries += jobs[i].n;
+                lp.max_slots = std::max(lp.max_slots, (cmax + 1) / 2);
                 lp.wide_uf = lp.wide_uf || maxpairs + 1 >= (uint64_t{1} << 31);
             } else {
                 lds.add(i, nodes, nodes, 1, desc_count, row_count, row_words, false);
@@ -581,11 +583,25 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                 if (spread) {
                     BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W, true>), grid(lp.hbm),
                               dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
-                              dcount + 2, chain_threshold, errs);
+                              dcount + 2, chain_threshold, 63u, 0u, errs);
                 } else {
-                    BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W, false>), grid(lp.hbm),
-                              dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
-                              dcount + 2, chain_threshold, errs);
+                    // Each XCD hooks its nodes one after another; a node whose
+                    // words exceed half an XCD's L2 (4 MB) but fit in it is
+                    // hooked in two passes of half the slots each, so that a
+                    // pass's random stores stay L2-resident.
+                    const uint64_t node_bytes = lp.max_slots * sizeof(W);
+                    const bool halves = node_bytes > (uint64_t{2} << 20) &&
+                                        node_bytes <= (uint64_t{4} << 20);
+                    uint32_t shift = 63;
+                    if (halves) {
+                        shift = 0;
+                        while ((uint64_t{2} << shift) < lp.max_slots) ++shift;
+                    }
+                    for (uint32_t part = 0; part < (halves ? 2u : 1u); ++part) {
+                        BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W, false>), grid(lp.hbm),
+                                  dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
+                                  dcount + 2, chain_threshold, shift, part, errs);
+                    }
                 }
                 DeferredEdge* const deferred = ws.deferred.as<DeferredEdge>();
                 BH_LAUNCH(prof, "k_union", (k_union<VI, W>), grid(lp.hbm), dim3(kBlock), 0, st,

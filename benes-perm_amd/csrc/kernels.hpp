@@ -354,6 +354,9 @@ __device__ __forceinline__ PairEdge pair_edge(const VI* inv, uint64_t off, uint6
     return e;
 }
 
+// Only slots s with (s >> part_shift) == part are written (a node whose
+// words would just overflow an XCD's L2 is hooked in two passes, each
+// writing half of them), so every slot is written by exactly one pass.
 // With kCount (depths whose nodes spread over several XCDs), every eighth
 // block counts its slots hooked under the slot just below them (`adjacent`),
 // up to just past `chain_threshold` (a wave adds its count only while the
@@ -368,6 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
                                                        const NodeDesc* __restrict__ desc,
                                                        unsigned long long* __restrict__ adjacent,
                                                        unsigned long long chain_threshold,
+                                                       uint32_t part_shift, uint32_t part,
                                                        const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
@@ -383,8 +387,12 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
         const uint64_t b = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q + 1));
         const uint64_t sa = in_slot(a, cnt), sb = in_slot(b, cnt);
         const W label = static_cast<W>(1 ^ ((a ^ b) & 1));
-        if (!(a & 1)) u[sa] = sb < sa ? (static_cast<W>(sb) << 1) | label : UfWord<W>::root;
-        if (!(b & 1)) u[sb] = sa < sb ? (static_cast<W>(sa) << 1) | label : UfWord<W>::root;
+        if (!(a & 1) && (sa >> part_shift) == part) {
+            u[sa] = sb < sa ? (static_cast<W>(sb) << 1) | label : UfWord<W>::root;
+        }
+        if (!(b & 1) && (sb >> part_shift) == part) {
+            u[sb] = sa < sb ? (static_cast<W>(sa) << 1) | label : UfWord<W>::root;
+        }
         if (kCount && (blockIdx.x & 7) == 0) {
             next_to = (!(a & 1) && sb + 1 == sa) || (!(b & 1) && sa + 1 == sb);
         }
@@ -392,7 +400,8 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
         // The unpaired output entry: no edge; an even entry there leaves its
         // slot a root.
         const uint64_t x = static_cast<uint64_t>(inv[off + cnt - 1]);
-        if (!(x & 1)) u[in_slot(x, cnt)] = UfWord<W>::root;
+        const uint64_t sx = in_slot(x, cnt);
+        if (!(x & 1) && (sx >> part_shift) == part) u[sx] = UfWord<W>::root;
     }
     if (!kCount || (blockIdx.x & 7) != 0) return;
     const uint64_t ballot = __ballot(next_to);
