@@ -49,6 +49,7 @@ EXPORTS = (
     "benes_hip_release_workspace",
     "benes_hip_autotune_report",
     "benes_hip_set_subtree_shape",
+    "benes_hip_set_hook_binning",
     "benes_hip_lookup",
     "benes_hip_lookup_device",
     "benes_hip_encoded_size",
@@ -178,6 +179,7 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_release_workspace.restype = None
     lib.benes_hip_autotune_report.argtypes = [cp, sz]
     lib.benes_hip_set_subtree_shape.argtypes = [u32, u32]
+    lib.benes_hip_set_hook_binning.argtypes = [ctypes.c_int]
     lib.benes_hip_compress_file.argtypes = [cp, cp, u32, fp, ctypes.POINTER(u64),
                                             ctypes.POINTER(ctypes.c_double), cp, sz]
     lib.benes_hip_lookup.argtypes = [vp, u64, vp, u64, ctypes.c_int, vp, u32, cp, sz]
@@ -516,6 +518,13 @@ def set_subtree_shape(log2_entries: int = 0, threads: int = 0) -> None:
     """Pins a subtree shape (one of SUBTREE_SHAPES); (0, 0) returns to tuning."""
     if library().benes_hip_set_subtree_shape(log2_entries, threads) != OK:
         raise ValueError(f"no subtree shape {log2_entries}x{threads}")
+
+
+def set_hook_binning(mode: int = -1) -> None:
+    """First-hook binning: -1 by size (default), 0 never, 1 wherever a node's
+    words exceed 4 MB (the blob is the same in every mode)."""
+    if library().benes_hip_set_hook_binning(mode) != OK:
+        raise ValueError(f"no hook binning mode {mode}")
 
 
 def release_workspace() -> None:

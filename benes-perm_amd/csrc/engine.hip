@@ -46,6 +46,10 @@ namespace {
 
 thread_local double g_device_seconds = 0;
 
+// First-hook binning: -1 by size (the default), 0 never, 1 wherever a node's
+// words overflow an XCD's L2 (a test setting).
+std::atomic<int> g_hook_binning{-1};
+
 // Optional per-kernel timing: an event pair around every launch, summed per
 // kernel name once the call's stream has drained.
 struct KernelStat {
@@ -96,7 +100,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     DevBuf input, level[2], ilevel[2], inv0, uf, srows, trows, desc, plans, items, segs, errs,
-        blob, deferred, deferred_counts;
+        blob, deferred, deferred_counts, bin_fill;
     std::vector<int> host_errs;
     PinnedRing ring;  // host <-> device staging of this workspace's stream
     Workspace() = default;
@@ -381,6 +385,7 @@ struct SegList {
 void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_t* out,
                    int shape) {
     const SubtreeShape sub = kShapes[shape];
+    const int hook_binning = g_hook_binning.load();
     const hipStream_t st = ws.stream;
     const uint32_t count = static_cast<uint32_t>(jobs.size());
     uint64_t total_n = 0, total_words = 0;
@@ -427,7 +432,11 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     const Launch per_entry = per_entry_list.commit(table);
     struct LevelPlan {
         Launch hbm, lds, pack, narrow;  // pack, narrow: both sides' segments
+        Launch bin;                     // k_hook_bin: the HBM nodes, 4096 pairs per block
         bool wide_uf = false;
+        bool binned = false;            // first hooks staged in bins of 2^bin_log2 words
+        uint32_t bin_log2 = kHookBinMinLog2;
+        uint64_t bins = 0, scatter_blocks = 0;
         uint64_t hbm_entries = 0;  // entries of the items coloured in HBM at this depth
         uint64_t max_slots = 0;    // input slots of the largest HBM node
     };
@@ -438,7 +447,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     uint64_t desc_base = 0, max_rows = 1;
     for (uint32_t d = 0; d < max_df; ++d) {
         LevelPlan& lp = levels[d];
-        SegList hbm, lds, pack, narrow;
+        SegList hbm, lds, pack, narrow, bin;
         uint64_t desc_count = desc_base, row_count = 0;
         const uint64_t nodes = uint64_t{1} << d;
         for (uint32_t i = 0; i < count; ++i) {
@@ -451,6 +460,10 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             if (cmax > kLevelLds) {
                 const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
                 hbm.add(i, nodes * chunks, nodes, chunks, desc_count, row_count, row_words, true);
+                const uint64_t bin_chunks =
+                    ceil_div(maxpairs + 1, uint64_t{kHookBinThreads} * kHookBinPairs);
+                bin.add(i, nodes * bin_chunks, nodes, bin_chunks, desc_count, row_count, row_words,
+                        true);
                 lp.hbm_entries += jobs[i].n;
                 lp.max_slots = std::max(lp.max_slots, (cmax + 1) / 2);
                 lp.wide_uf = lp.wide_uf || maxpairs + 1 >= (uint64_t{1} << 31);
@@ -478,6 +491,28 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
             row_count += nodes * row_words;
         }
         lp.hbm = hbm.commit(table);
+        // First hooks are binned where the direct stores are bound by address
+        // translation: nodes of at least 64 MB of 32-bit words in a level of
+        // at least 2^31 entries (measured at 2^26 and 2^32 + 12345 entries:
+        // binning gains nothing at 2^26, and 50 ms per depth at 2^32 where
+        // nodes hold 64 MB or more).  The test setting bins every node whose
+        // words overflow an XCD's L2.
+        const uint64_t node_bytes = lp.max_slots * 4;
+        lp.binned = lp.hbm.count && !lp.wide_uf &&
+                    (hook_binning == 1 ? node_bytes > (uint64_t{4} << 20)
+                     : hook_binning < 0 ? node_bytes >= (uint64_t{64} << 20) &&
+                                              total_n >= (uint64_t{1} << 31)
+                                        : false);
+        if (lp.binned) {
+            lp.bin = bin.commit(table);
+            // The smallest bins (from 1 MB) a node spans at most kHookBinSpan of.
+            while (ceil_div(lp.max_slots, uint64_t{1} << lp.bin_log2) + 1 > kHookBinSpan) {
+                ++lp.bin_log2;
+            }
+            lp.bins = ceil_div(total_n, uint64_t{1} << lp.bin_log2);
+            lp.scatter_blocks = ceil_div(lp.bins, kXcds) * kXcds *
+                                ((uint64_t{1} << lp.bin_log2) / kHookScatterEntries);
+        }
         lp.lds = lds.commit(table);
         lp.pack = pack.commit(table);
         lp.narrow = narrow.commit(table);
@@ -490,11 +525,14 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         subtree_list.add(i, uint64_t{1} << jobs[i].df, 0, 0, 0, 0, 0, false);
     }
     const Launch subtree = subtree_list.commit(table);
+    // Per-depth bin counters, room for the smallest bins.
+    const uint64_t hook_bins = ceil_div(total_n, uint64_t{1} << kHookBinMinLog2);
     // One dispatch holds at most 2^32 - 1 work-items.
     uint64_t most = std::max(subtree.blocks * sub.threads, describe_all.blocks * 64);
     for (const LevelPlan& lp : levels) {
         most = std::max({most, lp.hbm.blocks * kBlock, lp.lds.blocks * kLevelThreads,
-                         lp.pack.blocks * kBlock, lp.narrow.blocks * kNarrowThreads});
+                         lp.pack.blocks * kBlock, lp.narrow.blocks * kNarrowThreads,
+                         lp.bin.blocks * kHookBinThreads, lp.scatter_blocks * kBlock});
     }
     if (most >= (uint64_t{1} << 32)) {
         throw std::length_error("batch too large for one call: split it into smaller batches");
@@ -516,6 +554,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         ws.trows.ensure(8 * max_rows);
         ws.deferred.ensure(sizeof(DeferredEdge) * kDeferCap);
         ws.deferred_counts.ensure(sizeof(unsigned long long) * 4 * max_df);
+        ws.bin_fill.ensure(sizeof(uint32_t) * hook_bins * max_df);
     }
     BH_CHECK(hipMemcpyAsync(ws.plans.ptr, plans.data(), sizeof(Plan) * count,
                             hipMemcpyHostToDevice, st));
@@ -529,6 +568,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     if (max_df > 0) {
         BH_CHECK(hipMemsetAsync(ws.deferred_counts.ptr, 0, sizeof(unsigned long long) * 4 * max_df,
                                 st));
+        BH_CHECK(hipMemsetAsync(ws.bin_fill.ptr, 0, sizeof(uint32_t) * hook_bins * max_df, st));
     }
 
     const ItemCtx* ditems = ws.items.as<ItemCtx>();
@@ -580,7 +620,30 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                 const bool spread = d < 3;
                 const unsigned long long chain_threshold =
                     spread ? lp.hbm_entries / 2 / 64 / 8 : ~0ull;
-                if (spread) {
+                if constexpr (std::is_same_v<W, uint32_t>) {
+                    if (lp.binned) {
+                        // Staged in this depth's output level arrays, which
+                        // the routing kernels write only after the union.
+                        uint32_t* const fill = ws.bin_fill.as<uint32_t>() + hook_bins * d;
+                        if (spread) {
+                            BH_LAUNCH(prof, "k_hook_bin", (k_hook_bin<VI, true>), grid(lp.bin),
+                                      dim3(kHookBinThreads), 0, st, segp(lp.bin), lp.bin.count,
+                                      inv, desc, lp.bin_log2, pdst, idst, fill, dcount + 2,
+                                      chain_threshold, errs);
+                        } else {
+                            BH_LAUNCH(prof, "k_hook_bin", (k_hook_bin<VI, false>), grid(lp.bin),
+                                      dim3(kHookBinThreads), 0, st, segp(lp.bin), lp.bin.count,
+                                      inv, desc, lp.bin_log2, pdst, idst, fill, dcount + 2,
+                                      chain_threshold, errs);
+                        }
+                        BH_LAUNCH(prof, "k_hook_scatter", k_hook_scatter,
+                                  dim3(static_cast<uint32_t>(lp.scatter_blocks)), dim3(kBlock), 0,
+                                  st, uf, pdst, idst, fill, lp.bins, lp.bin_log2);
+                    }
+                }
+                if (lp.binned) {
+                    // Hooked above.
+                } else if (spread) {
                     BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W, true>), grid(lp.hbm),
                               dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
                               dcount + 2, chain_threshold, 63u, 0u, errs);
@@ -1495,6 +1558,12 @@ int benes_hip_set_subtree_shape(uint32_t log2_entries, uint32_t threads) {
     return BENES_HIP_INVALID_ARGUMENT;
 }
 
+int benes_hip_set_hook_binning(int mode) {
+    if (mode < -1 || mode > 1) return BENES_HIP_INVALID_ARGUMENT;
+    g_hook_binning.store(mode);
+    return BENES_HIP_OK;
+}
+
 int benes_hip_autotune_report(char* out, size_t len) {
     const std::string text = Tuner::instance().describe();
     if (out && len) {
@@ -1512,7 +1581,8 @@ void benes_hip_release_workspace(void) {
     if (ws.stream) (void)hipStreamSynchronize(ws.stream);
     for (DevBuf* b : {&ws.input, &ws.level[0], &ws.level[1], &ws.ilevel[0], &ws.ilevel[1],
                       &ws.inv0, &ws.uf, &ws.srows, &ws.trows, &ws.desc, &ws.plans, &ws.items,
-                      &ws.segs, &ws.errs, &ws.blob}) {
+                      &ws.segs, &ws.errs, &ws.blob, &ws.deferred, &ws.deferred_counts,
+                      &ws.bin_fill}) {
         b->release();
     }
 }

@@ -411,6 +411,162 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
     }
 }
 
+// First hooks of very large nodes, in two passes.  Stored straight into a
+// node of many megabytes of words within a union-find array of gigabytes,
+// each random first-hook store misses the address translation caches and
+// dirties a line that is written back before its neighbours arrive.
+// Instead, k_hook_bin stages each store as (word index within its bin, word)
+// in the bin of 2^bin_log2 words it falls in: a block sorts its stores by bin
+// in LDS, reserves a run in each bin with one atomic, and writes the runs out
+// contiguously.  k_hook_scatter then writes the bins, each by the blocks of
+// one XCD, so that a bin's words stay within one XCD's L2 and translation
+// caches while they are written.  Every slot is written once, so bin b
+// receives at most 2^bin_log2 stores and its run fits in positions
+// [b << bin_log2, (b + 1) << bin_log2) of the staging arrays, which are the
+// level arrays this depth writes only after its union-find is done.
+// `bin_fill[b]` counts bin b's stores (zero before the depth starts).
+constexpr uint32_t kHookBinMinLog2 = 18;  // 1 MB of 32-bit words
+constexpr uint32_t kHookBinThreads = 1024;
+constexpr uint32_t kHookBinPairs = 4;  // output pairs per thread
+constexpr uint32_t kHookBinStores = 2 * kHookBinThreads * kHookBinPairs;
+// Bins a node's slot words may touch, a multiple of 64 (one wave scans
+// them): a node of s slots touches at most ceil(s / 2^bin_log2) + 1.
+constexpr uint32_t kHookBinSpan = 320;
+constexpr uint32_t kHookScatterPerThread = 8;
+constexpr uint32_t kHookScatterEntries = kBlock * kHookScatterPerThread;  // per block
+
+template <typename VI, bool kCount>
+__global__ __launch_bounds__(kHookBinThreads) void k_hook_bin(
+    const Seg* __restrict__ segs, uint32_t nsegs, const VI* __restrict__ inv,
+    const NodeDesc* __restrict__ desc, uint32_t bin_log2, uint32_t* __restrict__ bin_word,
+    uint32_t* __restrict__ bin_value, uint32_t* __restrict__ bin_fill,
+    unsigned long long* __restrict__ adjacent, unsigned long long chain_threshold,
+    const int* __restrict__ errs) {
+    __shared__ uint32_t fill[kHookBinSpan];   // this block's stores per bin
+    __shared__ uint32_t start[kHookBinSpan];  // their run's first staging position
+    __shared__ uint32_t base[kHookBinSpan];   // their run's first position in the bin
+    __shared__ uint32_t stage_rel[kHookBinStores];
+    __shared__ uint32_t stage_val[kHookBinStores];
+    __shared__ uint32_t total;
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    uint64_t node, chunk;
+    if (!block_to_node(sg, node, chunk, false)) return;
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, cnt = nd.cnt, h = cnt >> 1;
+    // Staged word indices are relative to the node's first bin.
+    const uint64_t bin0 = off >> bin_log2;
+    const uint64_t rel0 = off - (bin0 << bin_log2);
+    for (uint32_t i = threadIdx.x; i < kHookBinSpan; i += kHookBinThreads) fill[i] = 0;
+    __syncthreads();
+    // Store j of this thread: slot 2k (the pair's first entry) or 2k + 1.
+    uint32_t rel[2 * kHookBinPairs], val[2 * kHookBinPairs], rank[2 * kHookBinPairs];
+    bool has[2 * kHookBinPairs];
+    uint32_t adjacent_hooks = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kHookBinPairs; ++k) {
+        const uint64_t q = (chunk * kHookBinPairs + k) * kHookBinThreads + threadIdx.x;
+        has[2 * k] = has[2 * k + 1] = false;
+        bool next_to = false;
+        if (q < h) {
+            const uint64_t a = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q));
+            const uint64_t b =
+                static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q + 1));
+            const uint64_t sa = in_slot(a, cnt), sb = in_slot(b, cnt);
+            const uint32_t label = static_cast<uint32_t>(1 ^ ((a ^ b) & 1));
+            if (!(a & 1)) {
+                has[2 * k] = true;
+                rel[2 * k] = static_cast<uint32_t>(rel0 + sa);
+                val[2 * k] = sb < sa ? (static_cast<uint32_t>(sb) << 1) | label
+                                     : UfWord<uint32_t>::root;
+            }
+            if (!(b & 1)) {
+                has[2 * k + 1] = true;
+                rel[2 * k + 1] = static_cast<uint32_t>(rel0 + sb);
+                val[2 * k + 1] = sa < sb ? (static_cast<uint32_t>(sa) << 1) | label
+                                         : UfWord<uint32_t>::root;
+            }
+            next_to = (!(a & 1) && sb + 1 == sa) || (!(b & 1) && sa + 1 == sb);
+        } else if ((cnt & 1) && q == h) {
+            const uint64_t x = static_cast<uint64_t>(inv[off + cnt - 1]);
+            if (!(x & 1)) {
+                has[2 * k] = true;
+                rel[2 * k] = static_cast<uint32_t>(rel0 + in_slot(x, cnt));
+                val[2 * k] = UfWord<uint32_t>::root;
+            }
+        }
+        if (kCount) adjacent_hooks += static_cast<uint32_t>(__popcll(__ballot(next_to)));
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 2 * kHookBinPairs; ++j) {
+        if (has[j]) rank[j] = atomicAdd(&fill[rel[j] >> bin_log2], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // Staging runs in bin order; one reservation per bin this block fills.
+        uint32_t carry = 0;
+        for (uint32_t c = 0; c < kHookBinSpan; c += 64) {
+            const uint32_t i = c + lane_id();
+            const uint32_t f = fill[i];
+            const uint32_t inc = static_cast<uint32_t>(wave_inclusive_sum(f));
+            start[i] = carry + inc - f;
+            if (f) base[i] = atomicAdd(bin_fill + bin0 + i, f);
+            carry += __shfl(inc, 63);
+        }
+        if (lane_id() == 0) total = carry;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < 2 * kHookBinPairs; ++j) {
+        if (has[j]) {
+            const uint32_t at = start[rel[j] >> bin_log2] + rank[j];
+            stage_rel[at] = rel[j];
+            stage_val[at] = val[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t mask = (uint32_t{1} << bin_log2) - 1;
+    for (uint32_t i = threadIdx.x; i < total; i += kHookBinThreads) {
+        const uint32_t r = stage_rel[i], lb = r >> bin_log2;
+        const uint64_t at = ((bin0 + lb) << bin_log2) + base[lb] + (i - start[lb]);
+        __builtin_nontemporal_store(r & mask, bin_word + at);
+        __builtin_nontemporal_store(stage_val[i], bin_value + at);
+    }
+    if (!kCount || (blockIdx.x & 7) != 0) return;
+    if (adjacent_hooks && lane_id() == 0 &&
+        __hip_atomic_load(adjacent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= chain_threshold) {
+        atomicAdd(adjacent, static_cast<unsigned long long>(adjacent_hooks));
+    }
+}
+
+// Bin b's stores, by the blocks of one XCD (block i runs on XCD i % 8).
+// Round r takes bins 8r .. 8r + 7, XCD x bin 8r + (x + r) % 8: the XCDs take
+// their bins one after another, and rotating the assignment keeps them
+// evenly loaded where only some bins of each node hold slots.
+__global__ __launch_bounds__(kBlock) void k_hook_scatter(uint32_t* __restrict__ uf,
+                                                         const uint32_t* __restrict__ bin_word,
+                                                         const uint32_t* __restrict__ bin_value,
+                                                         const uint32_t* __restrict__ bin_fill,
+                                                         uint64_t bins, uint32_t bin_log2) {
+    // Blocks per bin: 2^bin_log2 / kHookScatterEntries.
+    const uint64_t chunks = (uint64_t{1} << bin_log2) / kHookScatterEntries;
+    const uint64_t xcd = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+    const uint64_t round = local / chunks;
+    const uint64_t bin = round * kXcds + (xcd + round) % kXcds;
+    const uint64_t chunk = local % chunks;
+    if (bin >= bins) return;
+    const uint32_t n = bin_fill[bin];
+    const uint64_t first = bin << bin_log2;
+#pragma unroll
+    for (uint32_t k = 0; k < kHookScatterPerThread; ++k) {
+        const uint64_t e = (chunk * kHookScatterPerThread + k) * kBlock + threadIdx.x;
+        if (e < n) {
+            uf[first + __builtin_nontemporal_load(bin_word + first + e)] =
+                __builtin_nontemporal_load(bin_value + first + e);
+        }
+    }
+}
+
 // Rem walks longer than this many steps are handed to k_flatten.  A walk
 // follows one dependent load per step; on random
 // permutations the longest walk of a level is a few dozen steps, but the

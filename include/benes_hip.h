@@ -286,6 +286,14 @@ int benes_hip_autotune_report(char* out, size_t len);
  * choice back to the tuner with (0, 0).  No counterpart in the reference. */
 int benes_hip_set_subtree_shape(uint32_t log2_entries, uint32_t threads);
 
+/* First-hook binning of the union-find (staging the random first-hook stores
+ * of very large nodes in bins that one XCD then writes): -1 by size (the
+ * default: nodes of at least 64 MB of words in levels of at least 2^31
+ * entries), 0 never, 1 wherever a node's words exceed 4 MB.  The blob is the
+ * same in every mode; tests use 1 to check the binned path at small sizes.
+ * No counterpart in the reference. */
+int benes_hip_set_hook_binning(int mode);
+
 /* Frees the device buffers the library keeps for the calling thread's
  * current device (they grow to the largest call so far and are otherwise
  * kept for reuse).  No counterpart in the reference. */
