@@ -437,6 +437,8 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 2^32+12345 u64 run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the 2^32+12345 u64 leg, printed as its own JSON line (profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this backend compresses one permutation (or one "
@@ -446,6 +448,10 @@ def main():
     torch.cuda.init()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.large_only:
+        assert B.available(), "gfx950 backend unavailable"
+        print(json.dumps({"u64_2e32": large_u64()}), flush=True)
         return
     assert B.available(), "gfx950 backend unavailable"
     t0 = time.perf_counter()
