@@ -23,6 +23,9 @@
 //                    one thread per output pair: parity union-find over the
 //                    node's input slots in HBM (plain-store first hooks, then
 //                    one CAS per remaining hook; long walks deferred)
+//   k_hook_bin, k_hook_scatter
+//                    the first hooks of very large nodes instead: staged by
+//                    word bin, then written bin by bin, each bin by one XCD
 //   k_flatten        pointer-jumping flatten and the deferred edges, only
 //                    when k_union deferred one
 //   k_route_inputs   one thread per input pair: switch values, routing of P,
