@@ -525,8 +525,11 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         subtree_list.add(i, uint64_t{1} << jobs[i].df, 0, 0, 0, 0, 0, false);
     }
     const Launch subtree = subtree_list.commit(table);
-    // Per-depth bin counters, room for the smallest bins.
+    // Per-depth bin counters, room for the smallest bins (only when a depth
+    // bins its first hooks).
     const uint64_t hook_bins = ceil_div(total_n, uint64_t{1} << kHookBinMinLog2);
+    bool any_binned = false;
+    for (const LevelPlan& lp : levels) any_binned = any_binned || lp.binned;
     // One dispatch holds at most 2^32 - 1 work-items.
     uint64_t most = std::max(subtree.blocks * sub.threads, describe_all.blocks * 64);
     for (const LevelPlan& lp : levels) {
@@ -554,7 +557,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         ws.trows.ensure(8 * max_rows);
         ws.deferred.ensure(sizeof(DeferredEdge) * kDeferCap);
         ws.deferred_counts.ensure(sizeof(unsigned long long) * 4 * max_df);
-        ws.bin_fill.ensure(sizeof(uint32_t) * hook_bins * max_df);
+        if (any_binned) ws.bin_fill.ensure(sizeof(uint32_t) * hook_bins * max_df);
     }
     BH_CHECK(hipMemcpyAsync(ws.plans.ptr, plans.data(), sizeof(Plan) * count,
                             hipMemcpyHostToDevice, st));
@@ -568,7 +571,9 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     if (max_df > 0) {
         BH_CHECK(hipMemsetAsync(ws.deferred_counts.ptr, 0, sizeof(unsigned long long) * 4 * max_df,
                                 st));
-        BH_CHECK(hipMemsetAsync(ws.bin_fill.ptr, 0, sizeof(uint32_t) * hook_bins * max_df, st));
+        if (any_binned) {
+            BH_CHECK(hipMemsetAsync(ws.bin_fill.ptr, 0, sizeof(uint32_t) * hook_bins * max_df, st));
+        }
     }
 
     const ItemCtx* ditems = ws.items.as<ItemCtx>();
