@@ -712,6 +712,10 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restri
     const uint64_t p = chunk * kBlock + threadIdx.x;
     uint32_t s = 0;
     if (p < h) {
+        // The pair's entries are loaded first, so that their latency overlaps
+        // the find's dependent loads.
+        const uint64_t a0 = load_perm(it, plev, level0, off + 2 * p);
+        const uint64_t a1 = load_perm(it, plev, level0, off + 2 * p + 1);
         // The walk halves its path; then the start slot is pointed straight
         // at its root.  Slots are taken in ascending order within a node and
         // parents are smaller, so later walks soon meet such words.
@@ -724,8 +728,6 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restri
             const W r = uf_find<DeviceForest>(u, x0, s);
             if (r != x0) DeviceForest::store(u + x0, (r << 1) | static_cast<W>(s));
         }
-        const uint64_t a0 = load_perm(it, plev, level0, off + 2 * p);
-        const uint64_t a1 = load_perm(it, plev, level0, off + 2 * p + 1);
         __builtin_nontemporal_store(static_cast<uint32_t>((s ? a1 : a0) >> 1), pdst + off + p);
         __builtin_nontemporal_store(static_cast<uint32_t>((s ? a0 : a1) >> 1), pdst + off + lc + p);
     } else if ((cnt & 1) && p == h) {
