@@ -260,28 +260,35 @@ constexpr uint64_t kSpreadNodes = 64;
 
 __device__ __forceinline__ bool block_to_node_at(const Seg& sg, uint64_t block, uint64_t& node,
                                                  uint64_t& chunk, bool spread) {
-    const uint64_t b = block - sg.block0;
-    if (sg.nodes % kXcds != 0) {
+    // Every quantity here is below 2^32 (a launch has fewer than 2^32
+    // work-items), so the divisions are 32-bit ones, far shorter than 64-bit.
+    const uint32_t b = static_cast<uint32_t>(block - sg.block0);
+    const uint32_t nodes = static_cast<uint32_t>(sg.nodes);
+    const uint32_t chunks = static_cast<uint32_t>(sg.chunks);
+    uint32_t n, c;
+    if (nodes % kXcds != 0) {
         // 1, 2 or 4 nodes (the segment starts on a multiple of 8): block b
         // serves node b mod nodes, so each node keeps to its own XCDs.
-        node = b % sg.nodes;
-        chunk = b / sg.nodes;
+        n = b % nodes;
+        c = b / nodes;
     } else {
-        const uint64_t xcd = b % kXcds, local = b / kXcds, per_xcd = sg.nodes / kXcds;
+        const uint32_t xcd = b % kXcds, local = b / kXcds, per_xcd = nodes / kXcds;
         if (spread) {
             // At most kSpreadNodes of the XCD's nodes at a time, so that very
             // large items (thousands of nodes per XCD) keep the live words
             // within reach of the address translation caches.
-            const uint64_t g = per_xcd < kSpreadNodes ? per_xcd : kSpreadNodes;
-            const uint64_t grp = local / (g * sg.chunks), in_grp = local - grp * (g * sg.chunks);
-            node = (grp * g + in_grp % g) * kXcds + xcd;
-            chunk = in_grp / g;
+            const uint32_t g = per_xcd < kSpreadNodes ? per_xcd : static_cast<uint32_t>(kSpreadNodes);
+            const uint32_t grp = local / (g * chunks), in_grp = local - grp * (g * chunks);
+            n = (grp * g + in_grp % g) * kXcds + xcd;
+            c = in_grp / g;
         } else {
-            node = (local / sg.chunks) * kXcds + xcd;
-            chunk = local % sg.chunks;
+            n = (local / chunks) * kXcds + xcd;
+            c = local % chunks;
         }
     }
-    return node < sg.nodes && chunk < sg.chunks;
+    node = n;
+    chunk = c;
+    return n < nodes && c < chunks;
 }
 
 __device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uint64_t& chunk,
