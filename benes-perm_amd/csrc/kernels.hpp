@@ -559,10 +559,10 @@ __global__ __launch_bounds__(kBlock) void k_hook_scatter(uint32_t* __restrict__ 
                                                          const uint32_t* __restrict__ bin_fill,
                                                          uint64_t bins, uint32_t bin_log2) {
     // Blocks per bin: 2^bin_log2 / kHookScatterEntries.
-    const uint64_t chunks = (uint64_t{1} << bin_log2) / kHookScatterEntries;
-    const uint64_t xcd = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
-    const uint64_t round = local / chunks;
-    const uint64_t bin = round * kXcds + (xcd + round) % kXcds;
+    const uint32_t chunks = (uint32_t{1} << bin_log2) / kHookScatterEntries;
+    const uint32_t xcd = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+    const uint32_t round = local / chunks;
+    const uint64_t bin = uint64_t{round} * kXcds + (xcd + round) % kXcds;
     const uint64_t chunk = local % chunks;
     if (bin >= bins) return;
     const uint32_t n = bin_fill[bin];
@@ -937,12 +937,14 @@ __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ ite
     const uint64_t packets = sg.chunks;
     const SideDepth sd = P.side[side][d];
     const uint32_t r = sd.relative_level, g = sd.group_depth;
-    const uint64_t wpj = pack_words_per_job(g);
-    const uint64_t t = (blockIdx.x - sg.block0) * kBlock + threadIdx.x;
-    const uint64_t job = t / wpj, wi = t - job * wpj;
+    // A launch has fewer than 2^32 work-items: 32-bit divisions.
+    const uint32_t wpj = static_cast<uint32_t>(pack_words_per_job(g));
+    const uint32_t t = static_cast<uint32_t>((blockIdx.x - sg.block0) * kBlock + threadIdx.x);
+    const uint32_t job = t / wpj, wi = t - job * wpj;
     const uint32_t root_depth = d - r;
     const uint64_t groups = uint64_t{1} << root_depth;
-    const uint64_t group = job / packets, packet = job % packets;
+    const uint64_t group = job / static_cast<uint32_t>(packets);
+    const uint64_t packet = job % static_cast<uint32_t>(packets);
     if (group >= groups) return;
     const uint32_t span_log2 = g - r - 1;
     const uint64_t span = uint64_t{1} << span_log2;
