@@ -50,6 +50,7 @@ EXPORTS = (
     "benes_hip_autotune_report",
     "benes_hip_set_subtree_shape",
     "benes_hip_set_hook_binning",
+    "benes_hip_set_wide_words",
     "benes_hip_lookup",
     "benes_hip_lookup_device",
     "benes_hip_encoded_size",
@@ -180,6 +181,7 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_autotune_report.argtypes = [cp, sz]
     lib.benes_hip_set_subtree_shape.argtypes = [u32, u32]
     lib.benes_hip_set_hook_binning.argtypes = [ctypes.c_int]
+    lib.benes_hip_set_wide_words.argtypes = [ctypes.c_int]
     lib.benes_hip_compress_file.argtypes = [cp, cp, u32, fp, ctypes.POINTER(u64),
                                             ctypes.POINTER(ctypes.c_double), cp, sz]
     lib.benes_hip_lookup.argtypes = [vp, u64, vp, u64, ctypes.c_int, vp, u32, cp, sz]
@@ -525,6 +527,13 @@ def set_hook_binning(mode: int = -1) -> None:
     words exceed 4 MB (the blob is the same in every mode)."""
     if library().benes_hip_set_hook_binning(mode) != OK:
         raise ValueError(f"no hook binning mode {mode}")
+
+
+def set_wide_words(mode: int = 0) -> None:
+    """64-bit union-find words and level-0 inverse: 0 by size (default), 1 at
+    every level of every call (the blob is the same in both modes)."""
+    if library().benes_hip_set_wide_words(mode) != OK:
+        raise ValueError(f"no wide-words mode {mode}")
 
 
 def release_workspace() -> None:

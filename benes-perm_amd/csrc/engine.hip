@@ -50,6 +50,11 @@ thread_local double g_device_seconds = 0;
 // words overflow an XCD's L2 (a test setting).
 std::atomic<int> g_hook_binning{-1};
 
+// 64-bit union-find words and level-0 inverse: 0 by size (the default: words
+// where a node has 2^31 slots or more, the inverse from 2^32 - 1 entries),
+// 1 at every HBM level of every call (a test setting).
+std::atomic<int> g_wide_words{0};
+
 // Optional per-kernel timing: an event pair around every launch, summed per
 // kernel name once the call's stream has drained.
 struct KernelStat {
@@ -382,11 +387,13 @@ struct SegList {
 // in ws.errs.
 // `out` (8-byte aligned, total_words * 8 bytes) receives the blobs, or null
 // for the workspace's own blob buffer.
-void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_t* out,
-                   int shape) {
+// Work is queued on `st`; the deferred-edge list of each depth holds up to
+// `defer_cap` edges (at most kDeferCap).
+void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profiler& prof,
+                   uint8_t* out, int shape, uint64_t defer_cap) {
     const SubtreeShape sub = kShapes[shape];
     const int hook_binning = g_hook_binning.load();
-    const hipStream_t st = ws.stream;
+    const bool force_wide = g_wide_words.load() == 1;
     const uint32_t count = static_cast<uint32_t>(jobs.size());
     uint64_t total_n = 0, total_words = 0;
     uint32_t max_df = 0;
@@ -398,7 +405,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         total_n += j.n;
         total_words += ceil_div(j.lay.plan.end_offset, 8);
         max_df = std::max(max_df, j.df);
-        wide_inverse = wide_inverse || j.n >= 0xFFFFFFFFull;
+        wide_inverse = wide_inverse || force_wide || j.n >= 0xFFFFFFFFull;
     }
     // Device copies of plans and item contexts.
     ws.plans.ensure(sizeof(Plan) * count);
@@ -466,7 +473,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                         true);
                 lp.hbm_entries += jobs[i].n;
                 lp.max_slots = std::max(lp.max_slots, (cmax + 1) / 2);
-                lp.wide_uf = lp.wide_uf || maxpairs + 1 >= (uint64_t{1} << 31);
+                lp.wide_uf = lp.wide_uf || force_wide || maxpairs + 1 >= (uint64_t{1} << 31);
             } else {
                 lds.add(i, nodes, nodes, 1, desc_count, row_count, row_words, false);
             }
@@ -555,7 +562,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         ws.desc.ensure(sizeof(NodeDesc) * std::max<uint64_t>(1, desc_base));
         ws.srows.ensure(8 * max_rows);
         ws.trows.ensure(8 * max_rows);
-        ws.deferred.ensure(sizeof(DeferredEdge) * kDeferCap);
+        ws.deferred.ensure(sizeof(DeferredEdge) * defer_cap);
         ws.deferred_counts.ensure(sizeof(unsigned long long) * 4 * max_df);
         if (any_binned) ws.bin_fill.ensure(sizeof(uint32_t) * hook_bins * max_df);
     }
@@ -565,15 +572,18 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                             hipMemcpyHostToDevice, st));
     BH_CHECK(hipMemcpyAsync(ws.segs.ptr, table.data(), sizeof(Seg) * table.size(),
                             hipMemcpyHostToDevice, st));
-    BH_CHECK(hipMemsetAsync(ws.errs.ptr, 0, sizeof(int) * count, st));
-    BH_CHECK(hipMemsetAsync(blob_ptr, 0, total_words * 8, st));
-    BH_CHECK(hipMemsetAsync(ws.inv0.ptr, 0xFF, total_n * (wide_inverse ? 8 : 4), st));
+    auto fill = [&](void* p, uint64_t words, uint32_t value) {
+        if (words == 0) return;
+        const uint64_t blocks = std::min<uint64_t>(ceil_div(words, kBlock), uint64_t{1} << 16);
+        BH_LAUNCH(prof, "k_fill", k_fill_u32, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0,
+                  st, static_cast<uint32_t*>(p), words, value);
+    };
+    fill(ws.errs.ptr, count, 0);
+    fill(blob_ptr, total_words * 2, 0);
+    fill(ws.inv0.ptr, total_n * (wide_inverse ? 2 : 1), 0xFFFFFFFFu);
     if (max_df > 0) {
-        BH_CHECK(hipMemsetAsync(ws.deferred_counts.ptr, 0, sizeof(unsigned long long) * 4 * max_df,
-                                st));
-        if (any_binned) {
-            BH_CHECK(hipMemsetAsync(ws.bin_fill.ptr, 0, sizeof(uint32_t) * hook_bins * max_df, st));
-        }
+        fill(ws.deferred_counts.ptr, 2 * 4 * max_df, 0);
+        if (any_binned) fill(ws.bin_fill.ptr, hook_bins * max_df, 0);
     }
 
     const ItemCtx* ditems = ws.items.as<ItemCtx>();
@@ -583,12 +593,17 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
     auto grid = [](const Launch& l) { return dim3(static_cast<uint32_t>(l.blocks)); };
     auto segp = [&](const Launch& l) { return dsegs + l.first; };
 
+    // Launches of the 64-bit-word instantiations carry their own names, so
+    // that a profile shows which one ran.
     auto validate = [&](auto* inv) {
         using VI = std::remove_pointer_t<decltype(inv)>;
-        BH_LAUNCH(prof, "k_scatter_inverse", (k_scatter_inverse<VI>), grid(per_entry),
-                  dim3(kBlock), 0, st, ditems, segp(per_entry), per_entry.count, inv, errs);
-        BH_LAUNCH(prof, "k_check_inverse", (k_check_inverse<VI>), grid(per_entry), dim3(kBlock),
-                  0, st, ditems, segp(per_entry), per_entry.count, inv, errs);
+        constexpr bool w64 = sizeof(VI) == 8;
+        BH_LAUNCH(prof, w64 ? "k_scatter_inverse_w64" : "k_scatter_inverse",
+                  (k_scatter_inverse<VI>), grid(per_entry), dim3(kBlock), 0, st, ditems,
+                  segp(per_entry), per_entry.count, inv, errs);
+        BH_LAUNCH(prof, w64 ? "k_check_inverse_w64" : "k_check_inverse", (k_check_inverse<VI>),
+                  grid(per_entry), dim3(kBlock), 0, st, ditems, segp(per_entry), per_entry.count,
+                  inv, errs);
     };
     hipLaunchKernelGGL(k_headers, dim3(ceil_div(count, 64)), dim3(64), 0, st, ditems, count,
                        blob_ptr);
@@ -613,6 +628,7 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
         auto run = [&](auto* inv, auto* uf) {
             using VI = std::remove_const_t<std::remove_pointer_t<decltype(inv)>>;
             using W = std::remove_pointer_t<decltype(uf)>;
+            constexpr bool w64 = sizeof(W) == 8;
             if (lp.hbm.count) {
                 // Per depth: edges deferred, flatten blocks finished, hooks
                 // under the slot just below.
@@ -649,7 +665,8 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                 if (lp.binned) {
                     // Hooked above.
                 } else if (spread) {
-                    BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W, true>), grid(lp.hbm),
+                    BH_LAUNCH(prof, w64 ? "k_hook_first_w64" : "k_hook_first",
+                              (k_hook_first<VI, W, true>), grid(lp.hbm),
                               dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
                               dcount + 2, chain_threshold, 63u, 0u, errs);
                 } else {
@@ -666,34 +683,41 @@ void enqueue_batch(Workspace& ws, std::vector<Job>& jobs, Profiler& prof, uint8_
                         while ((uint64_t{2} << shift) < lp.max_slots) ++shift;
                     }
                     for (uint32_t part = 0; part < (halves ? 2u : 1u); ++part) {
-                        BH_LAUNCH(prof, "k_hook_first", (k_hook_first<VI, W, false>), grid(lp.hbm),
+                        BH_LAUNCH(prof, w64 ? "k_hook_first_w64" : "k_hook_first",
+                                  (k_hook_first<VI, W, false>), grid(lp.hbm),
                                   dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
                                   dcount + 2, chain_threshold, shift, part, errs);
                     }
                 }
                 DeferredEdge* const deferred = ws.deferred.as<DeferredEdge>();
-                BH_LAUNCH(prof, "k_union", (k_union<VI, W>), grid(lp.hbm), dim3(kBlock), 0, st,
-                          segp(lp.hbm), lp.hbm.count, inv, uf, desc, deferred, dcount, errs);
+                BH_LAUNCH(prof, w64 ? "k_union_w64" : "k_union", (k_union<VI, W>), grid(lp.hbm),
+                          dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc, deferred,
+                          dcount, defer_cap, errs);
                 // Returns at once unless k_union deferred an edge.
                 const dim3 flat_grid(static_cast<uint32_t>(std::min<uint64_t>(lp.hbm.blocks, 2048)));
                 if (spread) {
-                    BH_LAUNCH(prof, "k_flatten", (k_flatten<W, CoherentForest>), flat_grid,
-                              dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, lp.hbm.blocks, uf,
-                              desc, deferred, dcount, errs);
+                    BH_LAUNCH(prof, w64 ? "k_flatten_w64" : "k_flatten",
+                              (k_flatten<W, CoherentForest>), flat_grid, dim3(kBlock), 0, st,
+                              segp(lp.hbm), lp.hbm.count, lp.hbm.blocks, uf, desc, deferred,
+                              dcount, defer_cap, errs);
                 } else {
-                    BH_LAUNCH(prof, "k_flatten", (k_flatten<W, DeviceForest>), flat_grid,
-                              dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, lp.hbm.blocks, uf,
-                              desc, deferred, dcount, errs);
+                    BH_LAUNCH(prof, w64 ? "k_flatten_w64" : "k_flatten",
+                              (k_flatten<W, DeviceForest>), flat_grid, dim3(kBlock), 0, st,
+                              segp(lp.hbm), lp.hbm.count, lp.hbm.blocks, uf, desc, deferred,
+                              dcount, defer_cap, errs);
                 }
-                BH_LAUNCH(prof, "k_route_inputs", (k_route_inputs<W>), grid(lp.hbm), dim3(kBlock),
+                BH_LAUNCH(prof, w64 ? "k_route_inputs_w64" : "k_route_inputs", (k_route_inputs<W>),
+                          grid(lp.hbm), dim3(kBlock),
                           0, st, ditems, segp(lp.hbm), lp.hbm.count, level0, plev, pdst, uf, desc,
                           srows, dcount + 2, chain_threshold, errs);
-                BH_LAUNCH(prof, "k_route_outputs", (k_route_outputs<VI>), grid(lp.hbm),
+                BH_LAUNCH(prof, sizeof(VI) == 8 ? "k_route_outputs_w64" : "k_route_outputs",
+                          (k_route_outputs<VI>), grid(lp.hbm),
                           dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, idst, desc, srows,
                           trows, errs);
             }
             if (lp.lds.count) {
-                BH_LAUNCH(prof, "k_level_lds", (k_level_lds<VI>), grid(lp.lds),
+                BH_LAUNCH(prof, sizeof(VI) == 8 ? "k_level_lds_w64" : "k_level_lds",
+                          (k_level_lds<VI>), grid(lp.lds),
                           dim3(kLevelThreads), 0, st, ditems, segp(lp.lds), lp.lds.count, level0,
                           plev, inv, pdst, idst, desc, srows, trows, errs);
             }
@@ -850,7 +874,7 @@ void run_batch(Workspace& ws, std::vector<Job>& jobs, uint8_t* out, hipStream_t 
     bool trial = false;
     const int shape = Tuner::instance().pick(cls, trial);
     BH_CHECK(hipEventRecord(ws.ev_begin, s));
-    enqueue_batch(ws, jobs, prof, out, shape);
+    enqueue_batch(ws, s, jobs, prof, out, shape, kDeferCap);
     BH_CHECK(hipEventRecord(ws.ev_end, s));
     flags.assign(jobs.size(), 0);
     BH_CHECK(hipMemcpyAsync(flags.data(), ws.errs.ptr, sizeof(int) * jobs.size(),
@@ -1238,7 +1262,7 @@ void Server::process(Lane& lane) {
     bool trial = false;
     const int shape = Tuner::instance().pick(cls, trial);
     BH_CHECK(hipEventRecord(lane.ws.ev_begin, s));
-    enqueue_batch(lane.ws, lane.jobs, prof, nullptr, shape);
+    enqueue_batch(lane.ws, s, lane.jobs, prof, nullptr, shape, kDeferCap);
     BH_CHECK(hipEventRecord(lane.ws.ev_end, s));
     note_host_time("host:serve_enqueue", t0);
     std::vector<int> flags(batch.size());
@@ -1566,6 +1590,12 @@ int benes_hip_set_subtree_shape(uint32_t log2_entries, uint32_t threads) {
 int benes_hip_set_hook_binning(int mode) {
     if (mode < -1 || mode > 1) return BENES_HIP_INVALID_ARGUMENT;
     g_hook_binning.store(mode);
+    return BENES_HIP_OK;
+}
+
+int benes_hip_set_wide_words(int mode) {
+    if (mode < 0 || mode > 1) return BENES_HIP_INVALID_ARGUMENT;
+    g_wide_words.store(mode);
     return BENES_HIP_OK;
 }
 

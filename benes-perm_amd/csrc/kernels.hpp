@@ -223,6 +223,19 @@ __global__ __launch_bounds__(kBlock) void k_check_inverse(const ItemCtx* __restr
     }
 }
 
+// Fills `count` 32-bit words with `value`.  Scratch that must start at a
+// known value (the blob, the inverse, error flags and counters) is cleared
+// with these kernels rather than memset nodes, so a call queues only kernels
+// and copies on its stream.
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint64_t count,
+                                                      uint32_t value) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += stride) {
+        p[i] = value;
+    }
+}
+
 // Writes each item's header bytes (internal.hpp:431-454) at the start of its
 // zeroed blob, before any switch bit is OR'ed in.
 __global__ void k_headers(const ItemCtx* __restrict__ items, uint32_t count,
@@ -584,7 +597,8 @@ __global__ __launch_bounds__(kBlock) void k_hook_scatter(uint32_t* __restrict__ 
 // ones (a reverse of odd length, a rotation by one of even length), and the
 // edge closing such a cycle would walk half the node one load at a time.
 constexpr uint32_t kWalkCap = 1024;
-// Deferred edges per level; beyond this an edge walks its whole path itself.
+// Deferred edges per level (the list's capacity is a launch argument, at
+// most this); beyond it an edge walks its whole path itself.
 constexpr uint64_t kDeferCap = uint64_t{1} << 18;
 struct DeferredEdge {
     uint64_t off;       // the node's first union-find word
@@ -598,6 +612,7 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
                                                   const NodeDesc* __restrict__ desc,
                                                   DeferredEdge* __restrict__ deferred,
                                                   unsigned long long* __restrict__ deferred_count,
+                                                  uint64_t defer_cap,
                                                   const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
@@ -623,7 +638,7 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
     }
     // A long path: flattened for the edge by k_flatten, unless the list is full.
     const unsigned long long i = atomicAdd(deferred_count, 1ull);
-    if (i < kDeferCap) {
+    if (i < defer_cap) {
         deferred[i] = DeferredEdge{nd.off, e.lo | (static_cast<uint64_t>(e.label) << 63), e.hi};
         return;
     }
@@ -647,6 +662,7 @@ __global__ __launch_bounds__(kBlock) void k_flatten(const Seg* __restrict__ segs
                                                     const NodeDesc* __restrict__ desc,
                                                     const DeferredEdge* __restrict__ deferred,
                                                     unsigned long long* __restrict__ counts,
+                                                    uint64_t defer_cap,
                                                     const int* __restrict__ errs) {
     const unsigned long long total = counts[0];
     if (total == 0) return;
@@ -674,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void k_flatten(const Seg* __restrict__ segs
     __syncthreads();
     if (!last) return;
     __threadfence();
-    const uint64_t n = total < kDeferCap ? total : kDeferCap;
+    const uint64_t n = total < defer_cap ? total : defer_cap;
     for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
         const DeferredEdge e = deferred[i];
         const W lo = static_cast<W>(e.lo_label & ~(uint64_t{1} << 63));

@@ -294,6 +294,14 @@ int benes_hip_set_subtree_shape(uint32_t log2_entries, uint32_t threads);
  * No counterpart in the reference. */
 int benes_hip_set_hook_binning(int mode);
 
+/* 64-bit union-find words and level-0 inverse: 0 by size (the default:
+ * 64-bit words in nodes of 2^31 input slots or more, a 64-bit inverse from
+ * 2^32 - 1 entries), 1 at every level of every call.  The blob is the same in
+ * both modes; tests use 1 to check the 64-bit kernels at small sizes, as the
+ * reference's Metal backend does with BENES_METAL_FORCE_PARENT64
+ * (src/metal/benes_metal.mm:615-633). */
+int benes_hip_set_wide_words(int mode);
+
 /* Frees the device buffers the library keeps for the calling thread's
  * current device (they grow to the largest call so far and are otherwise
  * kept for reuse).  No counterpart in the reference. */
