@@ -36,6 +36,7 @@ EXPORTS = (
     "benes_hip_available",
     "benes_hip_compress",
     "benes_hip_compress_device",
+    "benes_hip_compress_device_noalloc",
     "benes_hip_compress_file",
     "benes_hip_batch_layout",
     "benes_hip_compress_batch",
@@ -182,6 +183,8 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_set_subtree_shape.argtypes = [u32, u32]
     lib.benes_hip_set_hook_binning.argtypes = [ctypes.c_int]
     lib.benes_hip_set_wide_words.argtypes = [ctypes.c_int]
+    lib.benes_hip_compress_device_noalloc.argtypes = [vp, u64, u32, fp, vp, u64,
+                                                      ctypes.POINTER(u64), vp, cp, sz]
     lib.benes_hip_compress_file.argtypes = [cp, cp, u32, fp, ctypes.POINTER(u64),
                                             ctypes.POINTER(ctypes.c_double), cp, sz]
     lib.benes_hip_lookup.argtypes = [vp, u64, vp, u64, ctypes.c_int, vp, u32, cp, sz]
@@ -284,6 +287,29 @@ def compress_device(d_perm_ptr: int, n: int, width: int, d_out_ptr: int, capacit
                                              ctypes.byref(f), ctypes.c_void_p(d_out_ptr),
                                              capacity, ctypes.byref(size),
                                              ctypes.c_void_p(stream or 0), err, 256)
+    _raise(rc, err)
+    return size.value
+
+
+# Limits of benes_hip_compress_device_noalloc (include/benes_hip.h).
+NOALLOC_MAX_ENTRIES = 1 << 20
+NOALLOC_SCRATCH_BYTES = 36 << 20
+
+
+def compress_device_noalloc(d_perm_ptr: int, n: int, width: int, d_out_ptr: int, capacity: int,
+                            fmt: FormatOptions | None = None, stream: int | None = None) -> int:
+    """compress_device without any device allocation: the library's fixed
+    per-device scratch holds every buffer (n <= NOALLOC_MAX_ENTRIES; d_out
+    8-byte aligned, capacity at least the blob size rounded up to 8).
+    Returns the blob size."""
+    f, keep = _format(fmt)
+    err = ctypes.create_string_buffer(256)
+    size = ctypes.c_uint64()
+    rc = library().benes_hip_compress_device_noalloc(ctypes.c_void_p(d_perm_ptr), n, width,
+                                                     ctypes.byref(f), ctypes.c_void_p(d_out_ptr),
+                                                     capacity, ctypes.byref(size),
+                                                     ctypes.c_void_p(stream or 0), err, 256)
+    del keep
     _raise(rc, err)
     return size.value
 

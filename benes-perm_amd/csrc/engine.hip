@@ -71,20 +71,43 @@ struct Pending {
     hipEvent_t begin, end;
 };
 
-// Grow-only device buffer.
+// A fixed range of device memory handed out front to back, reset before each
+// call that draws on it (the allocation-free entry point's scratch).
+struct Arena {
+    uint8_t* base = nullptr;
+    size_t cap = 0, used = 0;
+    void* take(size_t need) {
+        const size_t at = (used + 255) & ~size_t{255};
+        if (at > cap || need > cap - at) {
+            throw std::length_error("request needs more scratch than the allocation-free entry "
+                                    "point holds");
+        }
+        used = at + need;
+        return base + at;
+    }
+};
+
+// Grow-only device buffer, or (bound to an arena) a slice of the arena taken
+// anew by every ensure().
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    Arena* arena = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && !arena) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
     }
     void ensure(size_t need) {
+        if (arena) {
+            ptr = arena->take(need);
+            bytes = need;
+            return;
+        }
         if (need <= bytes) return;
         if (ptr) BH_CHECK(hipFree(ptr));
         ptr = nullptr;
@@ -125,6 +148,16 @@ struct Workspace {
         BH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         BH_CHECK(hipEventCreate(&ev_begin));
         BH_CHECK(hipEventCreate(&ev_end));
+    }
+    // Every device buffer draws on `a` from now on (no stream, events or
+    // pinned memory are made for such a workspace).
+    void bind(Arena* a) {
+        for (DevBuf* b : {&input, &level[0], &level[1], &ilevel[0], &ilevel[1], &inv0, &uf, &srows,
+                          &trows, &desc, &plans, &items, &segs, &errs, &blob, &deferred,
+                          &deferred_counts, &bin_fill}) {
+            b->release();
+            b->arena = a;
+        }
     }
 };
 
@@ -196,6 +229,16 @@ public:
             return e.issued++ % kNumShapes;
         }
         return best(e);
+    }
+    // The shape for a call that runs no trial (it is not timed): the pinned
+    // shape, the class's choice once its trials are done, else the first.
+    int settled(uint32_t cls) {
+        const int forced = forced_.load();
+        if (forced >= 0) return forced;
+        std::lock_guard<std::mutex> lock(mu_);
+        const auto it = table_.find(cls);
+        if (it == table_.end() || it->second.reported < BENES_HIP_AUTOTUNE_TRIAL_CALLS) return 0;
+        return best(it->second);
     }
     // Device time of a trial call; other calls are not recorded.
     void report(uint32_t cls, int shape, bool trial, double device_ms) {
@@ -1000,6 +1043,111 @@ int compress_one(const void* perm, bool on_device, uint64_t n, uint32_t width,
     return finish(st, err, err_len);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free compress (benes_hip_compress_device_noalloc): every device
+// buffer a call needs beside its input and output is a slice of `scratch`, a
+// fixed array that each device holds from the time the library's code object
+// is loaded on it until the process ends.  The host-side name of the array
+// is not a device address; the device's copy is located once per device with
+// hipGetSymbolAddress and only that address is used afterwards.  Calls on one
+// device share the array: a call clears what must start at known values
+// (blob, inverse, flags, counters) before its kernels run, and calls that
+// overlap on one device race.  Work goes on the caller's stream; no stream,
+// event, pinned or device memory is created.
+
+constexpr uint64_t kNoAllocMaxEntries = BENES_HIP_NOALLOC_MAX_ENTRIES;
+constexpr uint64_t kNoAllocScratchBytes = BENES_HIP_NOALLOC_SCRATCH_BYTES;
+// Deferred long walks per depth (an edge past it walks its own path).
+constexpr uint64_t kNoAllocDeferCap = uint64_t{1} << 14;
+
+// Worst case at 2^20 entries (the smallest subtree shape, 64-bit words
+// forced): 8 MiB inverse, four 4 MiB level arrays, 8 MiB of union-find
+// words, 384 KiB of deferred edges, under 300 KiB of rows, descriptors,
+// segments and plans.
+static_assert(kNoAllocScratchBytes >= 33 * (uint64_t{1} << 20) &&
+                  kNoAllocMaxEntries == (uint64_t{1} << 20),
+              "scratch sized for 2^20 entries");
+
+static __device__ __align__(256) uint64_t scratch[kNoAllocScratchBytes / 8];
+
+struct NoAllocState {
+    std::mutex mu;
+    bool ready = false;
+    uint8_t* ptr = nullptr;  // this device's copy of `scratch`
+    Arena arena;
+    Workspace ws;            // every buffer bound to the arena
+};
+
+NoAllocState& noalloc_for(int dev) {
+    static std::mutex mu;
+    static std::vector<NoAllocState*> all;
+    std::lock_guard<std::mutex> lock(mu);
+    if (static_cast<size_t>(dev) >= all.size()) all.resize(dev + 1, nullptr);
+    if (!all[dev]) all[dev] = new NoAllocState();
+    return *all[dev];
+}
+
+int compress_noalloc(const void* d_perm, uint64_t n, uint32_t width,
+                     const benes_hip_format* format, uint8_t* d_out, uint64_t capacity,
+                     uint64_t* out_size, hipStream_t stream, char* err, size_t err_len) {
+    Status st;
+    std::vector<Job> jobs(1);
+    if (!prepare_job(d_perm, n, width, format, jobs[0], st)) return finish(st, err, err_len);
+    const uint64_t end = jobs[0].lay.plan.end_offset;
+    if (out_size) *out_size = end;
+    if (n > kNoAllocMaxEntries) {
+        st.set(kLength, "permutation too large for the allocation-free entry point (at most 2^20 "
+                        "entries)");
+        return finish(st, err, err_len);
+    }
+    if (!d_perm) {
+        st.set(kInvalidArgument, "permutation pointer is null");
+        return finish(st, err, err_len);
+    }
+    if (!d_out || (reinterpret_cast<uintptr_t>(d_out) & 7) != 0 || capacity < round8(end)) {
+        st.set(kBufferTooSmall, "output buffer must be 8-byte aligned and hold the blob rounded up "
+                                "to whole 8-byte words");
+        return finish(st, err, err_len);
+    }
+    if (!benes_hip_available()) {
+        st.set(kRuntime, "hip backend is unavailable");
+        return finish(st, err, err_len);
+    }
+    const int dev = current_device(st);
+    if (dev < 0) return finish(st, err, err_len);
+    NoAllocState& na = noalloc_for(dev);
+    std::lock_guard<std::mutex> lock(na.mu);
+    try {
+        if (!na.ready) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                st.set(kRuntime, std::string("cannot locate the allocation-free scratch: ") +
+                                     hipGetErrorString(e));
+                return finish(st, err, err_len);
+            }
+            na.ptr = static_cast<uint8_t*>(p);
+            na.arena.base = na.ptr;
+            na.arena.cap = kNoAllocScratchBytes;
+            na.ws.bind(&na.arena);
+            na.ready = true;
+        }
+        na.arena.used = 0;
+        Profiler prof;  // never on here: its events would be allocations
+        const int shape = Tuner::instance().settled(Tuner::class_of(jobs));
+        enqueue_batch(na.ws, stream, jobs, prof, d_out, shape, kNoAllocDeferCap);
+        int flag = 0;
+        BH_CHECK(hipMemcpyAsync(&flag, na.ws.errs.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+        BH_CHECK(hipStreamSynchronize(stream));
+        if (flag) st.set(kInvalidArgument, kNotPermutation);
+    } catch (const HipError& e) {
+        st.set(kRuntime, std::string("hip error: ") + hipGetErrorString(e.code) + " in " + e.what);
+    } catch (const std::length_error& e) {
+        st.set(kLength, e.what());
+    }
+    return finish(st, err, err_len);
+}
+
 // Whole-range positional reads and writes of a file descriptor.
 bool pread_all(int fd, uint8_t* dst, uint64_t len, uint64_t off) {
     while (len) {
@@ -1419,6 +1567,14 @@ int benes_hip_compress_device(const void* d_perm, uint64_t n, uint32_t width_byt
                               size_t err_len) {
     return compress_one(d_perm, true, n, width_bytes, format, nullptr, d_out, out_capacity,
                         out_size, static_cast<hipStream_t>(stream), nullptr, err, err_len);
+}
+
+int benes_hip_compress_device_noalloc(const void* d_perm, uint64_t n, uint32_t width_bytes,
+                                      const benes_hip_format* format, uint8_t* d_out,
+                                      uint64_t out_capacity, uint64_t* out_size, void* stream,
+                                      char* err, size_t err_len) {
+    return compress_noalloc(d_perm, n, width_bytes, format, d_out, out_capacity, out_size,
+                            static_cast<hipStream_t>(stream), err, err_len);
 }
 
 int benes_hip_batch_layout(benes_hip_item* items, uint64_t count, uint64_t* total_bytes) {

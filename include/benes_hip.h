@@ -77,6 +77,31 @@ int benes_hip_compress_device(const void* d_perm, uint64_t n, uint32_t width_byt
                               uint8_t* d_out, uint64_t out_capacity, uint64_t* out_size,
                               void* stream, char* err, size_t err_len);
 
+/* Allocation-free variant of benes_hip_compress_device.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc, hipFree or other allocating routine and
+ * creates no stream or event: every device buffer the kernels need beside the
+ * input and output is a slice of one fixed array of
+ * BENES_HIP_NOALLOC_SCRATCH_BYTES per device, which the device holds from the
+ * time the library's code object is loaded on it until the process ends.
+ * Work is queued on `stream` (NULL: the default stream) and the call returns
+ * once the blob is complete.  Limits: n <= BENES_HIP_NOALLOC_MAX_ENTRIES
+ * (BENES_HIP_LENGTH beyond), `d_out` 8-byte aligned with out_capacity at
+ * least the blob size rounded up to a multiple of 8 (BENES_HIP_BUFFER_TOO_SMALL
+ * otherwise, with the blob size in *out_size).  Every call on one device
+ * shares that device's array and keeps its contents until the next call:
+ * calls whose device work overlapped would race, so the library holds a
+ * per-device lock from a call's first launch until its stream has drained,
+ * and calls from several host threads run one after another.  Bytes are those
+ * of benes_hip_compress_device; per-kernel profiling does not cover it. */
+#define BENES_HIP_NOALLOC_MAX_ENTRIES (1ull << 20)
+#define BENES_HIP_NOALLOC_SCRATCH_BYTES (36ull << 20)
+int benes_hip_compress_device_noalloc(const void* d_perm, uint64_t n, uint32_t width_bytes,
+                                      const benes_hip_format* format,
+                                      uint8_t* d_out, uint64_t out_capacity, uint64_t* out_size,
+                                      void* stream, char* err, size_t err_len);
+
 /* ---- Batches: many permutations of uneven sizes and widths in one call. ----
  * Every launch of the pipeline covers all items of the batch at once; the
  * bytes of each blob are the same as those of a plain compress of the item.
