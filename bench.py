@@ -78,26 +78,63 @@ def algorithmic_bytes(kernel: str, n: int, blob: int) -> float:
     return per_launch.get(kernel, 0.0)
 
 
-def cpu_baseline(perm: np.ndarray, want: str | None):
+def _median_seconds(fn, runs: int = 5):
+    """Median wall time of `runs` calls (steady clock around each call), as
+    the reference's benchmark times compress (tools/benchmark.cpp:54-68)."""
+    times, result = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        result = fn()
+        times.append(time.perf_counter() - t0)
+    return sorted(times)[len(times) // 2], times, result
+
+
+def cpu_baseline(runs: int = 5):
+    """The reference's portable_cpu encoder (oracle/_ref, compiled from the
+    reference sources) on this host's cores, median of `runs` runs each:
+    a 2^24-entry sample of the headline workload (the line's value), and the
+    2^20 default-format configuration on one thread and on every core."""
     import support as S
 
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    if S.ref_available():
-        t0 = time.perf_counter()
-        blob = S.ref_compress(perm, 5, B.EMBEDDED_INPUT, (11, 10), (10, 11), threads=cores)
-        dt = time.perf_counter() - t0
-        ok = want is None or hashlib.sha256(blob).hexdigest() == want
-        return {"value": perm.size / dt / 1e6, "unit": "Mentries/s", "cores": cores,
-                "kind": "reference",
-                "sample": f"full workload (2^26 entries) once, portable_cpu threads={cores}",
-                "seconds": dt, "byte_exact": ok}
-    sample = 1 << 22
-    p = B.generate(B.FISHER_YATES, sample, SEED)
-    t0 = time.perf_counter()
-    S.oracle_compress(p, 5, B.EMBEDDED_INPUT, (10, 9), (9, 10))
-    dt = time.perf_counter() - t0
-    return {"value": sample / dt / 1e6, "unit": "Mentries/s", "cores": 1, "kind": "port",
-            "sample": "2^22 entries, single-threaded oracle", "seconds": dt}
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    if not S.ref_available():
+        sample = 1 << 22
+        p = B.generate(B.FISHER_YATES, sample, SEED)
+        dt, times, _ = _median_seconds(
+            lambda: S.oracle_compress(p, 5, B.EMBEDDED_INPUT, (10, 9), (9, 10)), runs)
+        return {"value": sample / dt / 1e6, "unit": "Mentries/s", "cores": 1, "kind": "port",
+                "sample": f"2^22 entries, single-threaded oracle, median of {runs}",
+                "seconds": times}
+    # The headline workload's format scaled to 2^24 entries (19 routing
+    # levels: input clusters {10, 9}, output {9, 10}, ranks in the input section).
+    sample = 1 << 24
+    p = B.generate(B.FISHER_YATES, sample, 0x5EED0000 + sample)
+    dt, times, _ = _median_seconds(
+        lambda: S.ref_compress(p, 5, B.EMBEDDED_INPUT, (10, 9), (9, 10), threads=cores), runs)
+    del p
+    # Config 1: 2^20 entries, default format, seed 0x5EED0000 + n.
+    rec = fixture("cfg1-fy-2^20")
+    p1 = B.generate(rec["kind"], rec["n"], rec["seed"], rec["rate"], rec["width"])
+    cfg1 = {}
+    for threads in (1, cores):
+        t1, times1, blob = _median_seconds(lambda: S.ref_compress(p1, 5, 0, threads=threads), runs)
+        cfg1[f"threads_{threads}"] = {
+            "Mentries_per_s": p1.size / t1 / 1e6, "median_s": t1, "seconds": times1,
+            "byte_exact": hashlib.sha256(blob).hexdigest() == rec["sha256"]}
+    return {"value": sample / dt / 1e6, "unit": "Mentries/s", "cores": cores, "kind": "reference",
+            "cpu_model": cpu_model,
+            "sample": f"2^24-entry Fisher-Yates sample of the workload (input clusters {{10,9}}, "
+                      f"output {{9,10}}, embedded_input), reference portable_cpu threads={cores}, "
+                      f"median of {runs} runs",
+            "median_s": dt, "seconds": times,
+            "config1_2e20": {"workload": "2^20 uint32 Fisher-Yates, default format, separate",
+                             **cfg1}}
 
 
 def calibrate_copy(nbytes: int = 256 << 20, reps: int = 5):
@@ -192,7 +229,7 @@ def device_lookup(d_out, size: int, d_perm, reps: int = 3):
     return {"lookups": N, "ms": dt * 1e3, "Mlookups_per_s": N / dt / 1e6, "round_trip_exact": ok}
 
 
-def host_inclusive(perm: np.ndarray, want: str | None, reps: int = 3):
+def host_inclusive(perm: np.ndarray, want: str | None, reps: int = 5):
     """benes_hip_compress from a host span to a host blob (PCIe both ways):
     the reference's own call shape, reported beside the device-resident rate."""
     B.compress_into(perm, FMT)
@@ -508,7 +545,7 @@ def main():
 
     ms_per_step = elapsed / args.steps * 1e3
     line = {
-        "metric": "Mentries/s compressed (byte-exact), plain compress 2^26 uint32",
+        "metric": "Mentries/s compressed (byte-exact) at 2^26 u32 + batched/served vs plain; p50/p99",
         "value": N * args.steps / elapsed / 1e6,
         "unit": "Mentries/s",
         "n_gpus": 1,
@@ -542,18 +579,22 @@ def main():
         },
         "setup": {"generate_s": gen_s},
     }
+    # The same compress from a host span to a host blob (PCIe both ways): the
+    # reference's own call shape and timing definition, beside the
+    # HBM-resident value.
+    line["host_inclusive"] = host_inclusive(perm, want)
     if not args.no_extra:
         line["lookup_2e26"] = device_lookup(d_out, size, d_perm)
         line["inputs_2e26"] = other_inputs(d_out, size, stream, max(3, args.steps // 2))
         line["structured_2e26"] = structured_inputs(stream, 3)
-        line["host_inclusive"] = host_inclusive(perm, want)
         line["batched"] = batched(args.batch_items)
         line["served"] = served(args.requests, args.offered_rps)
         if not args.no_large:
             line["u64_2e32"] = large_u64()
     line["autotune"] = B.autotune_report().strip().splitlines()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(perm, want)
+        del perm
+        line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 
 

@@ -241,12 +241,19 @@ def compress_into(perm: np.ndarray, fmt: FormatOptions | None = None,
     if perm.dtype not in (np.uint32, np.uint64):
         raise TypeError("permutation must be uint32 or uint64")
     f, keep = _format(fmt)
-    if out is None:
-        out = np.empty(encoded_size(perm.size, fmt), np.uint8)
     lib = library()
     err = ctypes.create_string_buffer(256)
     size = ctypes.c_uint64()
     secs = ctypes.c_double()
+    if out is None:
+        # Size query through the compress entry point itself, so that errors
+        # come in the reference's order (the permutation before the format).
+        rc = lib.benes_hip_compress(perm.ctypes.data_as(ctypes.c_void_p), perm.size,
+                                    perm.dtype.itemsize, ctypes.byref(f), None, 0,
+                                    ctypes.byref(size), None, err, 256)
+        if rc != BUFFER_TOO_SMALL:
+            _raise(rc if rc != OK else LOGIC, err)
+        out = np.empty(size.value, np.uint8)
     rc = lib.benes_hip_compress(perm.ctypes.data_as(ctypes.c_void_p), perm.size,
                                 perm.dtype.itemsize, ctypes.byref(f),
                                 out.ctypes.data_as(ctypes.c_void_p), out.size,

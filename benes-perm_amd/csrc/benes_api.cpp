@@ -10,9 +10,9 @@
 #include "benes_hip.h"
 
 namespace benes {
-namespace {
+namespace detail {
 
-[[noreturn]] void rethrow(int code, const char* message) {
+[[noreturn]] void throw_status(int code, const char* message) {
     switch (code) {
     case BENES_HIP_INVALID_ARGUMENT: throw std::invalid_argument(message);
     case BENES_HIP_OVERFLOW: throw std::overflow_error(message);
@@ -22,6 +22,12 @@ namespace {
     default: throw std::runtime_error(message);
     }
 }
+
+}  // namespace detail
+
+namespace {
+
+[[noreturn]] void rethrow(int code, const char* message) { detail::throw_status(code, message); }
 
 benes_hip_format c_format(const FormatOptions& format) {
     benes_hip_format f{};
@@ -50,12 +56,16 @@ std::vector<std::byte> compress_hip(std::span<const Value> permutation,
     const benes_hip_format f = c_format(format);
     char err[256] = {0};
     std::uint64_t size = 0;
-    // Size query first: errors in the input size or format surface here.
-    int rc = permutation.empty()
-        ? BENES_HIP_INVALID_ARGUMENT
-        : benes_hip_encoded_size(permutation.size(), &f, &size, err, sizeof(err));
-    if (permutation.empty()) rethrow(rc, "permutation size is out of range");
-    if (rc != BENES_HIP_OK) rethrow(rc, err);
+    // A size query (no output buffer).  Errors come back in the reference's
+    // order: the input's size, then for a rejected format the permutation
+    // itself (checked on the device, src/benes.cpp:361-363), then the
+    // format; an acceptable request answers BENES_HIP_BUFFER_TOO_SMALL with
+    // the blob's size.
+    int rc = benes_hip_compress(permutation.data(), permutation.size(), sizeof(Value), &f,
+                                nullptr, 0, &size, nullptr, err, sizeof(err));
+    if (rc != BENES_HIP_BUFFER_TOO_SMALL) {
+        rethrow(rc == BENES_HIP_OK ? BENES_HIP_LOGIC : rc, rc == BENES_HIP_OK ? "size query failed" : err);
+    }
     std::vector<std::byte> out(static_cast<std::size_t>(size));
     rc = benes_hip_compress(permutation.data(), permutation.size(), sizeof(Value), &f,
                             reinterpret_cast<std::uint8_t*>(out.data()), out.size(), &size,

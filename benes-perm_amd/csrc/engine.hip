@@ -995,13 +995,91 @@ void run_lookup(Workspace& ws, const uint8_t* d_blob, uint64_t blob_size,
     if (user_stream) stream_after(user_stream, s);
 }
 
+// Whether the n entries at device address `dsrc` are a permutation of
+// 0..n-1: the validation kernels of a compress, alone, on stream `s`.
+bool device_is_permutation(Workspace& ws, hipStream_t s, const void* dsrc, uint64_t n,
+                           uint32_t width) {
+    const bool wide = n >= 0xFFFFFFFFull;
+    ItemCtx item{};
+    item.src = dsrc;
+    item.n = n;
+    item.width = width;
+    const uint64_t per_thread = std::max<uint64_t>(1, ceil_div(n, uint64_t{1} << 31));
+    Seg seg{};
+    seg.blocks = ceil_div(n, kBlock * per_thread);
+    ws.items.ensure(sizeof(ItemCtx));
+    ws.segs.ensure(sizeof(Seg));
+    ws.errs.ensure(sizeof(int));
+    ws.inv0.ensure(n * (wide ? 8 : 4));
+    BH_CHECK(hipMemcpyAsync(ws.items.ptr, &item, sizeof(item), hipMemcpyHostToDevice, s));
+    BH_CHECK(hipMemcpyAsync(ws.segs.ptr, &seg, sizeof(seg), hipMemcpyHostToDevice, s));
+    const auto fill = [&](void* p, uint64_t words, uint32_t value) {
+        const uint64_t blocks = std::min<uint64_t>(ceil_div(words, kBlock), uint64_t{1} << 16);
+        hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, s,
+                           static_cast<uint32_t*>(p), words, value);
+    };
+    fill(ws.errs.ptr, 1, 0);
+    fill(ws.inv0.ptr, n * (wide ? 2 : 1), 0xFFFFFFFFu);
+    const dim3 grid(static_cast<uint32_t>(seg.blocks));
+    const ItemCtx* it = ws.items.as<ItemCtx>();
+    const Seg* sg = ws.segs.as<Seg>();
+    if (wide) {
+        hipLaunchKernelGGL(k_scatter_inverse<uint64_t>, grid, dim3(kBlock), 0, s, it, sg, 1u,
+                           ws.inv0.as<uint64_t>(), ws.errs.as<int>());
+        hipLaunchKernelGGL(k_check_inverse<uint64_t>, grid, dim3(kBlock), 0, s, it, sg, 1u,
+                           ws.inv0.as<uint64_t>(), ws.errs.as<int>());
+    } else {
+        hipLaunchKernelGGL(k_scatter_inverse<uint32_t>, grid, dim3(kBlock), 0, s, it, sg, 1u,
+                           ws.inv0.as<uint32_t>(), ws.errs.as<int>());
+        hipLaunchKernelGGL(k_check_inverse<uint32_t>, grid, dim3(kBlock), 0, s, it, sg, 1u,
+                           ws.inv0.as<uint32_t>(), ws.errs.as<int>());
+    }
+    BH_CHECK(hipGetLastError());
+    int flag = 0;
+    BH_CHECK(hipMemcpyAsync(&flag, ws.errs.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    BH_CHECK(hipStreamSynchronize(s));
+    return flag == 0;
+}
+
+// The reference validates the permutation before it looks at the format
+// (src/benes.cpp:361-363, src/cuda/benes_cuda.cu:1158): when the format has
+// been rejected (`st` holds that error), the input is checked on the device
+// and a non-permutation reports that instead.
+void permutation_first(const void* perm, bool on_device, uint64_t n, uint32_t width,
+                       hipStream_t user_stream, Status& st) {
+    if (!perm) return;
+    Status vs;
+    bool bad = false;
+    with_workspace(vs, [&](Workspace& ws) {
+        const void* dsrc = perm;
+        if (!on_device) {
+            ws.input.ensure(n * width);
+            ws.ring.upload(ws.input.as<uint8_t>(),
+                           {HostPiece{static_cast<uint8_t*>(const_cast<void*>(perm)), 0, n * width}},
+                           ws.stream);
+            dsrc = ws.input.ptr;
+        }
+        if (user_stream) stream_after(ws.stream, user_stream);
+        bad = !device_is_permutation(ws, ws.stream, dsrc, n, width);
+        if (user_stream) stream_after(user_stream, ws.stream);
+    });
+    if (bad) {
+        st = Status();
+        st.set(kInvalidArgument, kNotPermutation);
+    }
+}
+
 int compress_one(const void* perm, bool on_device, uint64_t n, uint32_t width,
                  const benes_hip_format* format, uint8_t* dst_host, uint8_t* dst_dev,
                  uint64_t capacity, uint64_t* out_size, hipStream_t user_stream,
                  double* device_seconds, char* err, size_t err_len) {
     Status st;
     std::vector<Job> jobs(1);
-    if (!prepare_job(perm, n, width, format, jobs[0], st)) return finish(st, err, err_len);
+    if (!check_input(n, width, st)) return finish(st, err, err_len);
+    if (!prepare_job(perm, n, width, format, jobs[0], st)) {
+        permutation_first(perm, on_device, n, width, user_stream, st);
+        return finish(st, err, err_len);
+    }
     const uint64_t end = jobs[0].lay.plan.end_offset;
     if (out_size) *out_size = end;
     if ((!dst_host && !dst_dev) || capacity < end) {
@@ -1092,10 +1170,12 @@ int compress_noalloc(const void* d_perm, uint64_t n, uint32_t width,
                      uint64_t* out_size, hipStream_t stream, char* err, size_t err_len) {
     Status st;
     std::vector<Job> jobs(1);
-    if (!prepare_job(d_perm, n, width, format, jobs[0], st)) return finish(st, err, err_len);
-    const uint64_t end = jobs[0].lay.plan.end_offset;
+    if (!check_input(n, width, st)) return finish(st, err, err_len);
+    const bool format_ok = prepare_job(d_perm, n, width, format, jobs[0], st);
+    const uint64_t end = format_ok ? jobs[0].lay.plan.end_offset : 0;
     if (out_size) *out_size = end;
     if (n > kNoAllocMaxEntries) {
+        st = Status();
         st.set(kLength, "permutation too large for the allocation-free entry point (at most 2^20 "
                         "entries)");
         return finish(st, err, err_len);
@@ -1104,7 +1184,8 @@ int compress_noalloc(const void* d_perm, uint64_t n, uint32_t width,
         st.set(kInvalidArgument, "permutation pointer is null");
         return finish(st, err, err_len);
     }
-    if (!d_out || (reinterpret_cast<uintptr_t>(d_out) & 7) != 0 || capacity < round8(end)) {
+    if (format_ok &&
+        (!d_out || (reinterpret_cast<uintptr_t>(d_out) & 7) != 0 || capacity < round8(end))) {
         st.set(kBufferTooSmall, "output buffer must be 8-byte aligned and hold the blob rounded up "
                                 "to whole 8-byte words");
         return finish(st, err, err_len);
@@ -1133,6 +1214,15 @@ int compress_noalloc(const void* d_perm, uint64_t n, uint32_t width,
             na.ready = true;
         }
         na.arena.used = 0;
+        if (!format_ok) {
+            // The format was rejected: the permutation is checked first, as
+            // the reference does (src/benes.cpp:361-363).
+            if (!device_is_permutation(na.ws, stream, d_perm, n, width)) {
+                st = Status();
+                st.set(kInvalidArgument, kNotPermutation);
+            }
+            return finish(st, err, err_len);
+        }
         Profiler prof;  // never on here: its events would be allocations
         const int shape = Tuner::instance().settled(Tuner::class_of(jobs));
         enqueue_batch(na.ws, stream, jobs, prof, d_out, shape, kNoAllocDeferCap);

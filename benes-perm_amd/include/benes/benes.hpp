@@ -82,6 +82,13 @@ std::vector<std::byte> compress_compact_hip(std::span<const std::uint64_t> permu
                                             FormatOptions format);
 double last_device_seconds() noexcept;
 
+// Throws the exception class the reference uses for a C ABI status code
+// (BENES_HIP_INVALID_ARGUMENT -> std::invalid_argument, _OVERFLOW ->
+// std::overflow_error, _LENGTH -> std::length_error, _LOGIC ->
+// std::logic_error, _OUT_OF_RANGE -> std::out_of_range, anything else ->
+// std::runtime_error) with `message`.
+[[noreturn]] void throw_status(int code, const char* message);
+
 }  // namespace detail
 
 }  // namespace benes

@@ -1,0 +1,242 @@
+// SPDX-License-Identifier: Apache-2.0
+// Drop-in check of the C++ layer (benes-perm_amd/libbenes.so): the reference's
+// public compression API (include/benes/benes.hpp:47-61, dispatch at
+// src/benes.cpp:422-458) called as a user of the reference would call it.
+//
+//   test_benes_api errors
+//       Status codes -> exception classes (detail::throw_status), backend
+//       names and availability, and the refusal of backends this library
+//       does not serve.  Needs no GPU.
+//   test_benes_api errors-device
+//       The reference's input and format errors, in its order, through
+//       benes::compress on the GPU.
+//   test_benes_api compress <out> <kind> <n> <seed> <rate_ppm> <width> <g>
+//                  <placement> <input clusters> <output clusters> <backend>
+//       One benes::compress of a seeded permutation (benes_hip_generate)
+//       with Backend::automatic or Backend::hip; the blob is written to
+//       <out>.  Cluster lists are comma-separated, "-" for the default.
+//
+// Every check prints "ok <name>" or "FAIL <name>: <why>"; the exit status is
+// the number of failures.
+#include <benes/benes.hpp>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <fstream>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <typeinfo>
+#include <vector>
+
+#include "benes_hip.h"
+
+namespace {
+
+int failures = 0;
+
+void report(bool ok, const std::string& name, const std::string& why = "") {
+    if (ok) {
+        std::printf("ok %s\n", name.c_str());
+    } else {
+        std::printf("FAIL %s: %s\n", name.c_str(), why.c_str());
+        ++failures;
+    }
+}
+
+// The class name of the exception `body` throws, most specific first.
+std::string thrown_class(const std::function<void()>& body, std::string* what = nullptr) {
+    try {
+        body();
+    } catch (const std::invalid_argument& e) {
+        if (what) *what = e.what();
+        return "invalid_argument";
+    } catch (const std::length_error& e) {
+        if (what) *what = e.what();
+        return "length_error";
+    } catch (const std::out_of_range& e) {
+        if (what) *what = e.what();
+        return "out_of_range";
+    } catch (const std::logic_error& e) {
+        if (what) *what = e.what();
+        return "logic_error";
+    } catch (const std::overflow_error& e) {
+        if (what) *what = e.what();
+        return "overflow_error";
+    } catch (const std::runtime_error& e) {
+        if (what) *what = e.what();
+        return "runtime_error";
+    } catch (const std::exception& e) {
+        if (what) *what = e.what();
+        return "exception";
+    }
+    return "none";
+}
+
+void expect_throw(const std::string& name, const std::function<void()>& body,
+                  const std::string& cls, const std::string& message = "") {
+    std::string what;
+    const std::string got = thrown_class(body, &what);
+    const bool ok = got == cls && (message.empty() || what.find(message) != std::string::npos);
+    report(ok, name, "threw " + got + " (" + what + "), want " + cls +
+                         (message.empty() ? "" : " with \"" + message + "\""));
+}
+
+std::vector<std::uint8_t> parse_list(const char* text) {
+    std::vector<std::uint8_t> out;
+    if (std::strcmp(text, "-") == 0) return out;
+    for (const char* p = text; *p;) {
+        out.push_back(static_cast<std::uint8_t>(std::strtoul(p, const_cast<char**>(&p), 10)));
+        if (*p == ',') ++p;
+    }
+    return out;
+}
+
+void errors() {
+    using benes::Backend;
+    const struct {
+        int code;
+        const char* cls;
+    } codes[] = {{BENES_HIP_INVALID_ARGUMENT, "invalid_argument"},
+                 {BENES_HIP_OVERFLOW, "overflow_error"},
+                 {BENES_HIP_LENGTH, "length_error"},
+                 {BENES_HIP_LOGIC, "logic_error"},
+                 {BENES_HIP_RUNTIME, "runtime_error"},
+                 {BENES_HIP_OUT_OF_RANGE, "out_of_range"},
+                 {BENES_HIP_BUFFER_TOO_SMALL, "runtime_error"},
+                 {BENES_HIP_PENDING, "runtime_error"}};
+    for (const auto& c : codes) {
+        expect_throw("status " + std::to_string(c.code) + " -> " + c.cls,
+                     [&] { benes::detail::throw_status(c.code, "message text"); }, c.cls,
+                     "message text");
+    }
+    const struct {
+        Backend b;
+        const char* name;
+    } names[] = {{Backend::automatic, "automatic"}, {Backend::portable_cpu, "portable-cpu"},
+                 {Backend::metal, "metal"},         {Backend::cuda, "cuda"},
+                 {Backend::avx512, "avx512"},       {Backend::hip, "hip"}};
+    for (const auto& n : names) {
+        report(std::strcmp(benes::backend_name(n.b), n.name) == 0,
+               std::string("backend_name ") + n.name, benes::backend_name(n.b));
+    }
+    for (Backend b : {Backend::portable_cpu, Backend::metal, Backend::cuda, Backend::avx512}) {
+        report(!benes::backend_available(b),
+               std::string("backend_available(") + benes::backend_name(b) + ") is false");
+        const std::vector<std::uint32_t> perm{1, 0, 2};
+        benes::CompressOptions o;
+        o.backend = b;
+        expect_throw(std::string("compress with ") + benes::backend_name(b),
+                     [&] { (void)benes::compress(std::span<const std::uint32_t>(perm), o); },
+                     "runtime_error", std::string(benes::backend_name(b)) + " backend is unavailable");
+    }
+    const bool hip = benes::backend_available(Backend::hip);
+    report(hip == benes::backend_available(Backend::automatic), "automatic follows hip");
+    report(hip == (benes_hip_available() != 0), "backend_available(hip) is benes_hip_available");
+    if (!hip) {
+        const std::vector<std::uint32_t> perm{1, 0};
+        expect_throw("compress without a device",
+                     [&] { (void)benes::compress(std::span<const std::uint32_t>(perm)); },
+                     "runtime_error", "hip backend is unavailable");
+    }
+}
+
+void errors_device() {
+    const char* kNotPerm = "input must contain every value from zero through size minus one";
+    const std::vector<std::uint32_t> empty;
+    expect_throw("empty input",
+                 [&] { (void)benes::compress(std::span<const std::uint32_t>(empty)); },
+                 "invalid_argument", "permutation size is out of range");
+    const std::vector<std::uint32_t> dup{0, 1, 1, 3};
+    expect_throw("duplicate value",
+                 [&] { (void)benes::compress(std::span<const std::uint32_t>(dup)); },
+                 "invalid_argument", kNotPerm);
+    const std::vector<std::uint64_t> high{0, 1, 7};
+    expect_throw("value out of range, 64-bit",
+                 [&] { (void)benes::compress(std::span<const std::uint64_t>(high)); },
+                 "invalid_argument", kNotPerm);
+    benes::CompressOptions bad_g;
+    bad_g.format.middle_group_log2 = 9;
+    // The reference checks the permutation before the format.
+    expect_throw("duplicate value before a bad format",
+                 [&] { (void)benes::compress(std::span<const std::uint32_t>(dup), bad_g); },
+                 "invalid_argument", kNotPerm);
+    std::vector<std::uint32_t> perm(4097);
+    for (std::uint32_t i = 0; i < perm.size(); ++i) perm[i] = (i * 3 + 5) % 4097;  // 3 is prime to 4097 = 17 * 241
+    expect_throw("middle_group_log2 out of range",
+                 [&] { (void)benes::compress(std::span<const std::uint32_t>(perm), bad_g); },
+                 "invalid_argument", "middle_group_log2 must be between 1 and 5");
+    benes::CompressOptions short_clusters;
+    short_clusters.format.input_clusters = {1, 1};
+    expect_throw("clusters not spanning the depth",
+                 [&] { (void)benes::compress(std::span<const std::uint32_t>(perm), short_clusters); },
+                 "invalid_argument", "cluster depths do not span every routing level");
+    benes::CompressOptions ok;
+    ok.backend = benes::Backend::hip;
+    const auto blob = benes::compress(std::span<const std::uint32_t>(perm), ok);
+    report(blob.size() > 20 && benes::detail::last_device_seconds() > 0,
+           "compress on the device, last_device_seconds");
+    auto blob64 = benes::compress(
+        std::span<const std::uint64_t>(std::vector<std::uint64_t>(perm.begin(), perm.end())), ok);
+    report(blob64 == blob, "uint64 input gives the uint32 bytes");
+    auto direct = benes::detail::compress_compact_hip(std::span<const std::uint32_t>(perm),
+                                                      benes::FormatOptions{});
+    report(direct == blob, "detail::compress_compact_hip equals compress");
+}
+
+int compress(int argc, char** argv) {
+    if (argc != 13) {
+        std::fprintf(stderr, "compress needs 11 arguments\n");
+        return 100;
+    }
+    const char* out = argv[2];
+    const auto kind = static_cast<std::uint32_t>(std::strtoul(argv[3], nullptr, 10));
+    const std::uint64_t n = std::strtoull(argv[4], nullptr, 10);
+    const std::uint64_t seed = std::strtoull(argv[5], nullptr, 10);
+    const auto rate = static_cast<std::uint32_t>(std::strtoul(argv[6], nullptr, 10));
+    const auto width = static_cast<std::uint32_t>(std::strtoul(argv[7], nullptr, 10));
+    benes::CompressOptions o;
+    o.format.middle_group_log2 = static_cast<std::uint8_t>(std::strtoul(argv[8], nullptr, 10));
+    o.format.middle_placement =
+        static_cast<benes::MiddlePlacement>(std::strtoul(argv[9], nullptr, 10));
+    o.format.input_clusters = parse_list(argv[10]);
+    o.format.output_clusters = parse_list(argv[11]);
+    o.backend = std::strcmp(argv[12], "hip") == 0 ? benes::Backend::hip : benes::Backend::automatic;
+    std::vector<std::byte> blob;
+    if (width == 4) {
+        std::vector<std::uint32_t> perm(n);
+        if (benes_hip_generate(kind, n, seed, rate, perm.data(), 4) != BENES_HIP_OK) return 101;
+        blob = benes::compress(std::span<const std::uint32_t>(perm), o);
+    } else {
+        std::vector<std::uint64_t> perm(n);
+        if (benes_hip_generate(kind, n, seed, rate, perm.data(), 8) != BENES_HIP_OK) return 101;
+        blob = benes::compress(std::span<const std::uint64_t>(perm), o);
+    }
+    std::ofstream f(out, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(blob.data()), static_cast<std::streamsize>(blob.size()));
+    report(static_cast<bool>(f), std::string("compress ") + argv[4] + " via " + argv[12]);
+    return failures;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const std::string mode = argc > 1 ? argv[1] : "";
+    try {
+        if (mode == "errors") {
+            errors();
+        } else if (mode == "errors-device") {
+            errors_device();
+        } else if (mode == "compress") {
+            return compress(argc, argv);
+        } else {
+            std::fprintf(stderr, "usage: test_benes_api errors|errors-device|compress ...\n");
+            return 100;
+        }
+    } catch (const std::exception& e) {
+        report(false, "uncaught exception", e.what());
+    }
+    return failures;
+}

@@ -29,6 +29,20 @@ def binning():
     B.set_hook_binning(-1)
 
 
+@pytest.fixture(autouse=True)
+def binned_kernels_ran(request):
+    # Every compress test here must have gone through the binned kernels, not
+    # silently through k_hook_first.
+    B.set_profiling(True)
+    yield
+    stats = B.kernel_stats()
+    B.set_profiling(False)
+    if request.node.name.startswith("test_binning_modes"):
+        return
+    for kernel in ("k_hook_bin", "k_hook_scatter"):
+        assert stats.get(kernel, (0, 0))[1] > 0, f"{kernel} did not run: {sorted(stats)}"
+
+
 @pytest.mark.parametrize("n,g,pl,width", [((1 << 21) + 5, 5, 0, 4), ((1 << 22) + 3, 4, 2, 4),
                                           ((1 << 23) + 1, 5, 1, 8), (1 << 23, 3, 2, 4)])
 def test_binned_random_matches_oracle(n, g, pl, width):

@@ -203,6 +203,15 @@ def test_invalid_input():
         B.compress(perm, S.fmt(ic=(1, 1)))
     with pytest.raises(ValueError):
         B.compress(perm, S.fmt(g=6))
+    # The permutation is checked before the format, as the reference does
+    # (src/benes.cpp:361-363): a non-permutation with a bad format reports
+    # the permutation, on both entry points.
+    with pytest.raises(ValueError, match="every value from zero"):
+        B.compress(np.array([0, 1, 1], np.uint32), S.fmt(g=6))
+    with pytest.raises(ValueError, match="every value from zero"):
+        B.compress(big, S.fmt(ic=(1, 1)))
+    with pytest.raises(ValueError, match="middle_group_log2 must be between 1 and 5"):
+        B.compress(perm, S.fmt(g=6))
     # the device is still healthy afterwards
     assert B.compress(perm) == S.oracle_compress(perm)
 
