@@ -29,7 +29,7 @@ LIB_PATH = os.path.join(HERE, "libbenes_hip.so")
  OUT_OF_RANGE) = range(9)
 
 SEPARATE, EMBEDDED_INPUT, EMBEDDED_OUTPUT = 0, 1, 2
-FISHER_YATES, SINGLE_CYCLE, NEAR_IDENTITY, IDENTITY, REVERSE, KEYED = range(6)
+FISHER_YATES, SINGLE_CYCLE, NEAR_IDENTITY, IDENTITY, REVERSE, KEYED, UNIFORM_PARALLEL = range(7)
 
 # Every extern "C" symbol declared in include/benes_hip.h.
 EXPORTS = (
@@ -52,6 +52,7 @@ EXPORTS = (
     "benes_hip_set_subtree_shape",
     "benes_hip_set_hook_binning",
     "benes_hip_set_wide_words",
+    "benes_hip_set_colouring",
     "benes_hip_lookup",
     "benes_hip_lookup_device",
     "benes_hip_encoded_size",
@@ -183,6 +184,7 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_set_subtree_shape.argtypes = [u32, u32]
     lib.benes_hip_set_hook_binning.argtypes = [ctypes.c_int]
     lib.benes_hip_set_wide_words.argtypes = [ctypes.c_int]
+    lib.benes_hip_set_colouring.argtypes = [ctypes.c_int]
     lib.benes_hip_compress_device_noalloc.argtypes = [vp, u64, u32, fp, vp, u64,
                                                       ctypes.POINTER(u64), vp, cp, sz]
     lib.benes_hip_compress_file.argtypes = [cp, cp, u32, fp, ctypes.POINTER(u64),
@@ -567,6 +569,16 @@ def set_wide_words(mode: int = 0) -> None:
     every level of every call (the blob is the same in both modes)."""
     if library().benes_hip_set_wide_words(mode) != OK:
         raise ValueError(f"no wide-words mode {mode}")
+
+
+UNION_FIND, RULERS = 0, 1
+
+
+def set_colouring(mode: int = UNION_FIND) -> None:
+    """Colouring of the HBM levels: UNION_FIND (default) or RULERS, an
+    independent second colouring (orbit minima); the blob is the same."""
+    if library().benes_hip_set_colouring(mode) != OK:
+        raise ValueError(f"no colouring mode {mode}")
 
 
 def release_workspace() -> None:

@@ -55,6 +55,10 @@ std::atomic<int> g_hook_binning{-1};
 // 1 at every HBM level of every call (a test setting).
 std::atomic<int> g_wide_words{0};
 
+// Colouring of the HBM levels: 0 the union-find (the default), 1 the ruler
+// orbit minima (an independent second colouring; the blob is the same).
+std::atomic<int> g_colouring{0};
+
 // Optional per-kernel timing: an event pair around every launch, summed per
 // kernel name once the call's stream has drained.
 struct KernelStat {
@@ -128,7 +132,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     DevBuf input, level[2], ilevel[2], inv0, uf, srows, trows, desc, plans, items, segs, errs,
-        blob, deferred, deferred_counts, bin_fill;
+        blob, deferred, deferred_counts, bin_fill, rulers;
     std::vector<int> host_errs;
     PinnedRing ring;  // host <-> device staging of this workspace's stream
     Workspace() = default;
@@ -154,7 +158,7 @@ struct Workspace {
     void bind(Arena* a) {
         for (DevBuf* b : {&input, &level[0], &level[1], &ilevel[0], &ilevel[1], &inv0, &uf, &srows,
                           &trows, &desc, &plans, &items, &segs, &errs, &blob, &deferred,
-                          &deferred_counts, &bin_fill}) {
+                          &deferred_counts, &bin_fill, &rulers}) {
             b->release();
             b->arena = a;
         }
@@ -437,6 +441,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
     const SubtreeShape sub = kShapes[shape];
     const int hook_binning = g_hook_binning.load();
     const bool force_wide = g_wide_words.load() == 1;
+    const bool rulers = g_colouring.load() == 1;
     const uint32_t count = static_cast<uint32_t>(jobs.size());
     uint64_t total_n = 0, total_words = 0;
     uint32_t max_df = 0;
@@ -489,6 +494,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         uint64_t bins = 0, scatter_blocks = 0;
         uint64_t hbm_entries = 0;  // entries of the items coloured in HBM at this depth
         uint64_t max_slots = 0;    // input slots of the largest HBM node
+        uint32_t ruler_rounds = 0; // pointer-jumping rounds covering the longest ruler cycle
     };
     std::vector<LevelPlan> levels(max_df);
     // Descriptors of all depths live side by side (depth d's from desc_base),
@@ -516,6 +522,10 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                         true);
                 lp.hbm_entries += jobs[i].n;
                 lp.max_slots = std::max(lp.max_slots, (cmax + 1) / 2);
+                // A node meets at most ceil(cmax / 32) + 1 ruler buckets.
+                uint32_t r = 0;
+                while ((uint64_t{1} << r) < ceil_div(cmax, uint64_t{1} << kRulerLog2) + 1) ++r;
+                lp.ruler_rounds = std::max(lp.ruler_rounds, r);
                 lp.wide_uf = lp.wide_uf || force_wide || maxpairs + 1 >= (uint64_t{1} << 31);
             } else {
                 lds.add(i, nodes, nodes, 1, desc_count, row_count, row_words, false);
@@ -591,6 +601,8 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         throw std::length_error("batch too large for one call: split it into smaller batches");
     }
 
+    // Ruler buckets of the level arrays (the ruler colouring).
+    const uint64_t ruler_buckets = ceil_div(total_n, uint64_t{1} << kRulerLog2) + 1;
     // Scratch.
     ws.segs.ensure(sizeof(Seg) * std::max<size_t>(1, table.size()));
     ws.inv0.ensure(total_n * (wide_inverse ? 8 : 4));
@@ -608,6 +620,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         ws.deferred.ensure(sizeof(DeferredEdge) * defer_cap);
         ws.deferred_counts.ensure(sizeof(unsigned long long) * 4 * max_df);
         if (any_binned) ws.bin_fill.ensure(sizeof(uint32_t) * hook_bins * max_df);
+        if (rulers) ws.rulers.ensure((sizeof(uint64_t) + sizeof(uint32_t)) * 2 * ruler_buckets);
     }
     BH_CHECK(hipMemcpyAsync(ws.plans.ptr, plans.data(), sizeof(Plan) * count,
                             hipMemcpyHostToDevice, st));
@@ -672,7 +685,35 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
             using VI = std::remove_const_t<std::remove_pointer_t<decltype(inv)>>;
             using W = std::remove_pointer_t<decltype(uf)>;
             constexpr bool w64 = sizeof(W) == 8;
-            if (lp.hbm.count) {
+            if (lp.hbm.count && rulers) {
+                // Rulers' walks, pointer jumping, then switch values and the
+                // routing of P; the owner marks use the union-find words.
+                uint32_t* const owner = ws.uf.as<uint32_t>();
+                uint64_t* const rmin0 = ws.rulers.as<uint64_t>();
+                uint64_t* const rmin1 = rmin0 + ruler_buckets;
+                uint32_t* const rnext0 = reinterpret_cast<uint32_t*>(rmin1 + ruler_buckets);
+                uint32_t* const rnext1 = rnext0 + ruler_buckets;
+                fill(owner, total_n, kUnowned);
+                fill(rnext0, ruler_buckets, kRulerUnwalked);
+                BH_LAUNCH(prof, "k_ruler_walk", (k_ruler_walk<VI>), grid(lp.hbm), dim3(kBlock), 0,
+                          st, ditems, segp(lp.hbm), lp.hbm.count, level0, plev, inv, desc, owner,
+                          rnext0, rmin0, errs);
+                const dim3 jump_grid(static_cast<uint32_t>(
+                    std::min<uint64_t>(ceil_div(ruler_buckets, kBlock), uint64_t{1} << 16)));
+                for (uint32_t r = 0; r < lp.ruler_rounds; ++r) {
+                    BH_LAUNCH(prof, "k_ruler_jump", k_ruler_jump, jump_grid, dim3(kBlock), 0, st,
+                              ruler_buckets, r & 1 ? rnext1 : rnext0, r & 1 ? rmin1 : rmin0,
+                              r & 1 ? rnext0 : rnext1, r & 1 ? rmin0 : rmin1);
+                }
+                const bool odd_rounds = (lp.ruler_rounds & 1) != 0;
+                BH_LAUNCH(prof, "k_ruler_route", (k_ruler_route<VI>), grid(lp.hbm), dim3(kBlock),
+                          0, st, ditems, segp(lp.hbm), lp.hbm.count, level0, plev, inv, pdst, desc,
+                          owner, odd_rounds ? rnext1 : rnext0, odd_rounds ? rmin1 : rmin0, srows,
+                          errs);
+                BH_LAUNCH(prof, sizeof(VI) == 8 ? "k_route_outputs_w64" : "k_route_outputs",
+                          (k_route_outputs<VI>), grid(lp.hbm), dim3(kBlock), 0, st, segp(lp.hbm),
+                          lp.hbm.count, inv, idst, desc, srows, trows, errs);
+            } else if (lp.hbm.count) {
                 // Per depth: edges deferred, flatten blocks finished, hooks
                 // under the slot just below.
                 unsigned long long* const dcount =
@@ -1839,6 +1880,12 @@ int benes_hip_set_hook_binning(int mode) {
     return BENES_HIP_OK;
 }
 
+int benes_hip_set_colouring(int mode) {
+    if (mode < 0 || mode > 1) return BENES_HIP_INVALID_ARGUMENT;
+    g_colouring.store(mode);
+    return BENES_HIP_OK;
+}
+
 int benes_hip_set_wide_words(int mode) {
     if (mode < 0 || mode > 1) return BENES_HIP_INVALID_ARGUMENT;
     g_wide_words.store(mode);
@@ -1863,7 +1910,7 @@ void benes_hip_release_workspace(void) {
     for (DevBuf* b : {&ws.input, &ws.level[0], &ws.level[1], &ws.ilevel[0], &ws.ilevel[1],
                       &ws.inv0, &ws.uf, &ws.srows, &ws.trows, &ws.desc, &ws.plans, &ws.items,
                       &ws.segs, &ws.errs, &ws.blob, &ws.deferred, &ws.deferred_counts,
-                      &ws.bin_fill}) {
+                      &ws.bin_fill, &ws.rulers}) {
         b->release();
     }
 }

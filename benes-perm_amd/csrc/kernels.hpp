@@ -796,6 +796,218 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict_
     if (lane_id() == 0 && first < h) trows[row + (first >> 6)] = mt;
 }
 
+// ---------------------------------------------------------------------------
+// Second colouring of the HBM levels, independent of the union-find: orbit
+// minima by rulers.  It replaces k_hook_first / k_union / k_flatten /
+// k_route_inputs when selected, and is there to check the union-find
+// colouring byte for byte at sizes no CPU encoder reaches.
+//
+// Along next(x) = I[P[x] ^ 1] ^ 1 (through x's output pair to the other
+// entry there, then to that entry's input-pair partner) the colour stays the
+// same: each step crosses one output pair and one input pair, each of which
+// joins entries of different colours.  A component of the colouring graph
+// (benes.cpp:247-297) is therefore two orbits of next(), one per colour,
+// holding one entry of each input pair of the component.  In a cycle the
+// smallest entry is even (it is 2s for the smallest pair s, whose partner
+// 2s + 1 is the smallest entry of the other orbit), and the reference gives
+// it colour 0, so colour(x) = (smallest entry of x's orbit) & 1.  In an odd
+// node the tail input t = c - 1 (no input partner) and the unpaired output
+// c - 1 (no output partner) end two chains instead: the one from t, which the
+// reference colours 0 first, stops where an entry reaches the unpaired
+// output; the other, colour 1, stops where next() would reach t.
+//
+// Rulers sample the orbits: one entry per aligned 32 entries of the level
+// array (its offset in the bucket hashed from the bucket), numbered by the
+// bucket.  Each ruler walks forward to the next ruler or a chain end,
+// recording the segment's smallest entry and marking the entries it passed
+// as its own; pointer jumping over the rulers (each segment of a cycle holds
+// at most one bucket's ruler, so ceil(log2(rulers per node)) rounds cover a
+// whole cycle) gives each ruler its orbit's minimum or chain end.  An entry
+// no ruler passed walks forward itself until it meets a ruler or a marked
+// entry (whose colour it shares), a chain end, or itself again (an orbit
+// without rulers: a short cycle, whose minimum it has then seen).
+constexpr uint32_t kRulerLog2 = 5;
+constexpr uint32_t kRulerEndA = 0xFFFFFFFFu;     // chain to the unpaired output: colour 0
+constexpr uint32_t kRulerEndB = 0xFFFFFFFEu;     // chain to the tail input: colour 1
+constexpr uint32_t kRulerUnwalked = 0xFFFFFFFDu; // bucket whose ruler no walk started
+constexpr uint32_t kUnowned = 0xFFFFFFFFu;       // entry no ruler passed
+constexpr uint64_t kOrbitEndA = ~uint64_t{0};
+constexpr uint64_t kOrbitEndB = ~uint64_t{1};
+
+__device__ __forceinline__ bool is_ruler(uint64_t abs) {
+    uint64_t z = (abs >> kRulerLog2) * 0x9E3779B97F4A7C15ull;
+    z ^= z >> 29;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 32;
+    return (abs & ((1u << kRulerLog2) - 1)) == (static_cast<uint32_t>(z) & ((1u << kRulerLog2) - 1));
+}
+
+// next(x) within the node at `off` of `cnt` entries, or a chain end.
+template <typename VI>
+__device__ __forceinline__ uint64_t orbit_next(const ItemCtx& it, const uint32_t* plev, bool level0,
+                                               const VI* inv, uint64_t off, uint64_t cnt,
+                                               uint64_t x) {
+    const uint64_t y = level0 ? load_src(it, off + x - it.base) : static_cast<uint64_t>(plev[off + x]);
+    if ((cnt & 1) && y == cnt - 1) return kOrbitEndA;
+    const uint64_t z = static_cast<uint64_t>(inv[off + (y ^ 1)]);
+    if ((cnt & 1) && z == cnt - 1) return kOrbitEndB;
+    return z ^ 1;
+}
+
+// Thread per input pair (both entries) and the tail: the walks of the rulers.
+template <typename VI>
+__global__ __launch_bounds__(kBlock) void k_ruler_walk(const ItemCtx* __restrict__ items,
+                                                       const Seg* __restrict__ segs, uint32_t nsegs,
+                                                       int level0, const uint32_t* __restrict__ plev,
+                                                       const VI* __restrict__ inv,
+                                                       const NodeDesc* __restrict__ desc,
+                                                       uint32_t* __restrict__ owner,
+                                                       uint32_t* __restrict__ rnext,
+                                                       uint64_t* __restrict__ rmin,
+                                                       const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    uint64_t node, chunk;
+    if (!block_to_node(sg, node, chunk, false)) return;
+    const ItemCtx it = items[sg.item];
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, cnt = nd.cnt, h = cnt >> 1;
+    const uint64_t q = chunk * kBlock + threadIdx.x;
+    const uint32_t first = q < h ? 0u : ((cnt & 1) && q == h ? 2u : 3u);
+    for (uint32_t k = first; k < 3; ++k) {
+        const uint64_t x = k < 2 ? 2 * q + k : cnt - 1;
+        const uint64_t abs = off + x;
+        if (!is_ruler(abs)) {
+            if (k == 1) break;
+            continue;
+        }
+        const uint32_t id = static_cast<uint32_t>(abs >> kRulerLog2);
+        uint64_t cur = x, m = x;
+        uint32_t nxt;
+        while (true) {
+            const uint64_t v = orbit_next(it, plev, level0 != 0, inv, off, cnt, cur);
+            if (v == kOrbitEndA) {
+                nxt = kRulerEndA;
+                break;
+            }
+            if (v == kOrbitEndB) {
+                nxt = kRulerEndB;
+                break;
+            }
+            if (is_ruler(off + v)) {
+                nxt = static_cast<uint32_t>((off + v) >> kRulerLog2);
+                break;
+            }
+            owner[off + v] = id;
+            m = v < m ? v : m;
+            cur = v;
+        }
+        owner[abs] = id;
+        rnext[id] = nxt;
+        rmin[id] = m;
+        if (k == 1) break;
+    }
+}
+
+// One pointer-jumping round over the rulers (thread per bucket).
+__global__ __launch_bounds__(kBlock) void k_ruler_jump(uint64_t buckets,
+                                                       const uint32_t* __restrict__ next_in,
+                                                       const uint64_t* __restrict__ min_in,
+                                                       uint32_t* __restrict__ next_out,
+                                                       uint64_t* __restrict__ min_out) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < buckets;
+         i += stride) {
+        const uint32_t j = next_in[i];
+        const uint64_t m = min_in[i];
+        if (j >= kRulerUnwalked) {
+            next_out[i] = j;
+            min_out[i] = m;
+            continue;
+        }
+        const uint64_t mj = min_in[j];
+        next_out[i] = next_in[j];
+        min_out[i] = mj < m ? mj : m;
+    }
+}
+
+__device__ __forceinline__ uint32_t ruler_colour(const uint32_t* rnext, const uint64_t* rmin,
+                                                 uint32_t id) {
+    const uint32_t n = rnext[id];
+    if (n == kRulerEndA) return 0;
+    if (n == kRulerEndB) return 1;
+    return static_cast<uint32_t>(rmin[id] & 1);
+}
+
+// Thread per input pair: s_p = colour(2p) from the rulers (an entry no ruler
+// passed walks to one), child routing of P, s ballot-packed into the node's
+// row, as k_route_inputs does from the union-find.
+template <typename VI>
+__global__ __launch_bounds__(kBlock) void k_ruler_route(const ItemCtx* __restrict__ items,
+                                                        const Seg* __restrict__ segs, uint32_t nsegs,
+                                                        int level0, const uint32_t* __restrict__ plev,
+                                                        const VI* __restrict__ inv,
+                                                        uint32_t* __restrict__ pdst,
+                                                        const NodeDesc* __restrict__ desc,
+                                                        const uint32_t* __restrict__ owner,
+                                                        const uint32_t* __restrict__ rnext,
+                                                        const uint64_t* __restrict__ rmin,
+                                                        uint64_t* __restrict__ srows,
+                                                        const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    uint64_t node, chunk;
+    if (!block_to_node(sg, node, chunk, false)) return;
+    const ItemCtx it = items[sg.item];
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, cnt = nd.cnt, row = nd.row;
+    const uint64_t h = cnt >> 1, lc = h + (cnt & 1);
+    const uint64_t p = chunk * kBlock + threadIdx.x;
+    uint32_t s = 0;
+    if (p < h) {
+        const uint64_t x = 2 * p;
+        const uint32_t o = owner[off + x];
+        if (is_ruler(off + x)) {
+            s = ruler_colour(rnext, rmin, static_cast<uint32_t>((off + x) >> kRulerLog2));
+        } else if (o != kUnowned) {
+            s = ruler_colour(rnext, rmin, o);
+        } else {
+            uint64_t cur = x, m = x;
+            while (true) {
+                const uint64_t v = orbit_next(it, plev, level0 != 0, inv, off, cnt, cur);
+                if (v == kOrbitEndA || v == kOrbitEndB) {
+                    s = v == kOrbitEndA ? 0u : 1u;
+                    break;
+                }
+                if (v == x) {
+                    s = static_cast<uint32_t>(m & 1);
+                    break;
+                }
+                if (is_ruler(off + v)) {
+                    s = ruler_colour(rnext, rmin, static_cast<uint32_t>((off + v) >> kRulerLog2));
+                    break;
+                }
+                const uint32_t ov = owner[off + v];
+                if (ov != kUnowned) {
+                    s = ruler_colour(rnext, rmin, ov);
+                    break;
+                }
+                m = v < m ? v : m;
+                cur = v;
+            }
+        }
+        const uint64_t a0 = load_perm(it, plev, level0, off + 2 * p);
+        const uint64_t a1 = load_perm(it, plev, level0, off + 2 * p + 1);
+        pdst[off + p] = static_cast<uint32_t>((s ? a1 : a0) >> 1);
+        pdst[off + lc + p] = static_cast<uint32_t>((s ? a0 : a1) >> 1);
+    } else if ((cnt & 1) && p == h) {
+        pdst[off + h] = static_cast<uint32_t>(load_perm(it, plev, level0, off + cnt - 1) >> 1);
+    }
+    const uint64_t ms = __ballot(s != 0);
+    const uint64_t first = p - lane_id();
+    if (lane_id() == 0 && first < h) srows[row + (first >> 6)] = ms;
+}
+
 // One workgroup per node of at most kLevelLds entries: the same three steps
 // as k_union / k_route_inputs / k_route_outputs, with the node's union-find
 // words and its input switch values in LDS instead of HBM.  A node of 2^16

@@ -279,6 +279,10 @@ uint64_t benes_hip_address_bit(const benes_hip_address* address, uint64_t pair);
 #define BENES_HIP_PERM_IDENTITY 3
 #define BENES_HIP_PERM_REVERSE 4
 #define BENES_HIP_PERM_KEYED 5 /* keyed Feistel bijection walked into [0, n); parallel */
+#define BENES_HIP_PERM_UNIFORM_PARALLEL 6 /* uniform, drawn on all host threads: each index to
+                                           * one of 2^k buckets by its own draw, buckets laid
+                                           * out in order, each shuffled by Fisher-Yates
+                                           * (Rao-Sandelius); host only */
 
 /* Writes n entries of width_bytes (4 or 8) into `out` (host memory).  The
  * generator is splitmix64 with Lemire's bounded draw, so the sequence is fixed
@@ -326,6 +330,16 @@ int benes_hip_set_hook_binning(int mode);
  * reference's Metal backend does with BENES_METAL_FORCE_PARENT64
  * (src/metal/benes_metal.mm:615-633). */
 int benes_hip_set_wide_words(int mode);
+
+/* Colouring of the levels coloured in HBM (nodes of more than 2^16 entries):
+ * 0 the parity union-find (the default), 1 ruler orbit minima, a second
+ * colouring that shares no code with the union-find: each entry's colour is
+ * read off the smallest entry of its orbit under x -> I[P[x] ^ 1] ^ 1, found
+ * by sampled rulers and pointer jumping.  Both give the reference's
+ * canonical colours (src/benes.cpp:278-297), so the blob is the same; mode 1
+ * is slower and exists to check the union-find byte for byte at sizes no CPU
+ * encoder reaches.  No counterpart in the reference. */
+int benes_hip_set_colouring(int mode);
 
 /* Frees the device buffers the library keeps for the calling thread's
  * current device (they grow to the largest call so far and are otherwise

@@ -18,9 +18,15 @@ do not depend on size:
     the ranks, and sampled entries decode to themselves;
   * a keyed pseudo-random bijection: sampled entries decode, forward and
     backward, to the values that were compressed (the oracle's decoder), and
-    the device lookup decodes all 4.3e9 entries back to the input.
-Inputs are generated on the device; the permutation (34 GB) never crosses to
-the host, the blob (16.5 GB) does once per case for the oracle's decoder.
+    the device lookup decodes all 4.3e9 entries back to the input;
+  * a uniformly random permutation: its blob equals, byte for byte, the blob
+    of a second colouring that shares no code with the union-find (the
+    rulers), which checks every switch bit as canonical, and sampled entries
+    decode.
+The structured and keyed inputs are generated on the device (the
+permutation, 34 GB, never crosses to the host), the uniform one on the
+host's threads; the blob (16.5 GB) crosses to the host once per case for the
+oracle's decoder.
 """
 from __future__ import annotations
 
@@ -163,3 +169,49 @@ def test_keyed_round_trip(big):
     progress("keyed: whole blob decoded on the device")
     assert torch.equal(d_dec, d_perm)
     del d_dec
+
+
+def equal_in_slices(a, b, step=1 << 30) -> bool:
+    return all(torch.equal(a[i:i + step], b[i:i + step]) for i in range(0, a.numel(), step))
+
+
+def test_uniform_two_colourings(big):
+    """A uniformly random permutation of the full size (drawn on the host's
+    threads by Rao-Sandelius bucketing and per-bucket Fisher-Yates: every
+    arrangement equally likely, as a seeded Fisher-Yates draw would give),
+    compressed with the union-find colouring and again with the ruler
+    colouring, which shares no code with it and is pinned against the oracle
+    and the reference digests by tests/test_gpu_rulers.py.  The two blobs
+    must be byte-identical: every switch bit of every level, coloured by the
+    64-bit union-find at the root and the 32-bit one below, is the canonical
+    one.  Sampled entries then decode through the oracle's decoder."""
+    d_perm, d_out, size, _ = big
+    progress("uniform: drawing the permutation on the host")
+    t0 = time.perf_counter()
+    host = B.generate(B.UNIFORM_PARALLEL, N, 0x5EED0000 + 32, 0, 8)
+    progress(f"uniform: drawn in {time.perf_counter() - t0:.1f} s")
+    d_perm.copy_(torch.from_numpy(host.view(np.int64)))
+    torch.cuda.synchronize()
+    progress("uniform: compressing with the union-find")
+    assert B.compress_device(d_perm.data_ptr(), N, 8, d_out.data_ptr(), d_out.numel(), FMT) == size
+    torch.cuda.synchronize()
+    d_ref = torch.empty_like(d_out)
+    B.set_colouring(B.RULERS)
+    try:
+        progress("uniform: compressing with the rulers")
+        t0 = time.perf_counter()
+        assert B.compress_device(d_perm.data_ptr(), N, 8, d_ref.data_ptr(), d_ref.numel(),
+                                 FMT) == size
+        torch.cuda.synchronize()
+        progress(f"uniform: ruler compress {time.perf_counter() - t0:.1f} s")
+    finally:
+        B.set_colouring(B.UNION_FIND)
+    same = equal_in_slices(d_out[:size], d_ref[:size])
+    del d_ref
+    assert same, "union-find and ruler colourings differ at 2^32 + 12345"
+    progress("uniform: blobs identical")
+    blob = d_out[:size].cpu().numpy()
+    idx = sample_indices(3)
+    assert np.array_equal(S.oracle_lookup(blob, idx), host[idx])
+    assert np.array_equal(S.oracle_lookup(blob, host[idx], inverse=True), idx)
+    progress("uniform: sampled lookups done")
