@@ -4,6 +4,7 @@
 // value, as detail::compress_compact_cuda does (src/internal.hpp:1193-1198).
 #include <benes/benes.hpp>
 
+#include <fstream>
 #include <stdexcept>
 #include <string>
 
@@ -92,15 +93,47 @@ std::vector<std::byte> compress(std::span<const std::uint64_t> permutation,
     return dispatch(permutation, std::move(options));
 }
 
+// The reference's compress_file (src/benes.cpp:498-553), in its order and with
+// its messages: open and size the input, read it whole, decode the
+// little-endian values, compress them (through benes_hip_compress, so every
+// compress error comes as compress raises it), and only then create or
+// truncate the output and write the blob.
 void compress_file(const std::filesystem::path& input, const std::filesystem::path& output,
                    InputValueWidth input_width, CompressOptions options) {
-    require_hip(options.backend);
-    const benes_hip_format f = c_format(options.format);
-    char err[256] = {0};
-    const int rc = benes_hip_compress_file(input.c_str(), output.c_str(),
-                                           static_cast<std::uint32_t>(input_width), &f, nullptr,
-                                           nullptr, err, sizeof(err));
-    if (rc != BENES_HIP_OK) rethrow(rc, err);
+    std::ifstream stream(input, std::ios::binary | std::ios::ate);
+    if (!stream) throw std::runtime_error("cannot open permutation input file");
+    const std::streamoff length = stream.tellg();
+    const std::size_t value_bytes = static_cast<std::size_t>(input_width);
+    if ((value_bytes != 4 && value_bytes != 8) || length <= 0 ||
+        static_cast<std::uint64_t>(length) % value_bytes != 0) {
+        throw std::invalid_argument("permutation input has an invalid element width or size");
+    }
+    stream.seekg(0);
+    std::vector<unsigned char> raw(static_cast<std::size_t>(length));
+    stream.read(reinterpret_cast<char*>(raw.data()), length);
+    if (!stream) throw std::runtime_error("failed to read permutation input file");
+    std::vector<std::byte> encoded;
+    const std::size_t count = raw.size() / value_bytes;
+    auto load = [&](std::size_t i) {
+        std::uint64_t v = 0;
+        for (std::size_t b = value_bytes; b-- > 0;) v = (v << 8) | raw[i * value_bytes + b];
+        return v;
+    };
+    if (input_width == InputValueWidth::bits32) {
+        std::vector<std::uint32_t> permutation(count);
+        for (std::size_t i = 0; i < count; ++i) permutation[i] = static_cast<std::uint32_t>(load(i));
+        raw = {};
+        encoded = compress(std::span<const std::uint32_t>(permutation), std::move(options));
+    } else {
+        std::vector<std::uint64_t> permutation(count);
+        for (std::size_t i = 0; i < count; ++i) permutation[i] = load(i);
+        raw = {};
+        encoded = compress(std::span<const std::uint64_t>(permutation), std::move(options));
+    }
+    std::ofstream out(output, std::ios::binary | std::ios::trunc);
+    out.write(reinterpret_cast<const char*>(encoded.data()),
+              static_cast<std::streamsize>(encoded.size()));
+    if (!out) throw std::runtime_error("failed to write compressed permutation file");
 }
 
 bool backend_available(Backend backend) noexcept {

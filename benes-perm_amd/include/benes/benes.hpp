@@ -7,7 +7,7 @@
 // Backend::automatic resolves to hip, and any other backend reports itself
 // unavailable exactly as the reference does for a backend it was built
 // without (src/benes.cpp:437-442).  compress_file is the reference's
-// (src/benes.cpp:498-553).  Lookup, views and file readers stay with the
+// (src/benes.cpp:498-553), over compress().  Lookup, views and file readers stay with the
 // reference library, which reads the byte-identical blobs this produces.
 //
 // detail::hip_available / detail::compress_compact_hip are the backend
@@ -64,8 +64,10 @@ struct CompressOptions {
 [[nodiscard]] std::vector<std::byte> compress(std::span<const std::uint64_t> permutation,
                                               CompressOptions options = {});
 
-// Raw little-endian values from `input`, the blob to `output` (streamed
-// through pinned memory to and from the GPU, with no host copy of either).
+// Raw little-endian values from `input`, the blob to `output`: the
+// reference's compress_file (src/benes.cpp:498-553), reading the whole file,
+// compressing it with compress() and writing the blob, with the reference's
+// errors in its order.
 void compress_file(const std::filesystem::path& input, const std::filesystem::path& output,
                    InputValueWidth input_width = InputValueWidth::bits64,
                    CompressOptions options = {});

@@ -10,6 +10,10 @@
 //   test_benes_api errors-device
 //       The reference's input and format errors, in its order, through
 //       benes::compress on the GPU.
+//   test_benes_api file <in> <out> <width> <g> <placement>
+//       benes::compress_file of a raw little-endian file (reference
+//       src/benes.cpp:498-553); prints "ok file" or the exception's class and
+//       message as "threw <class>: <message>".
 //   test_benes_api compress <out> <kind> <n> <seed> <rate_ppm> <width> <g>
 //                  <placement> <input clusters> <output clusters> <backend>
 //       One benes::compress of a seeded permutation (benes_hip_generate)
@@ -186,6 +190,27 @@ void errors_device() {
     report(direct == blob, "detail::compress_compact_hip equals compress");
 }
 
+int file(int argc, char** argv) {
+    if (argc != 7) {
+        std::fprintf(stderr, "file needs 5 arguments\n");
+        return 100;
+    }
+    benes::CompressOptions o;
+    o.format.middle_group_log2 = static_cast<std::uint8_t>(std::strtoul(argv[5], nullptr, 10));
+    o.format.middle_placement =
+        static_cast<benes::MiddlePlacement>(std::strtoul(argv[6], nullptr, 10));
+    const auto width = static_cast<benes::InputValueWidth>(std::strtoul(argv[4], nullptr, 10));
+    std::string what;
+    const std::string cls = thrown_class([&] { benes::compress_file(argv[2], argv[3], width, o); },
+                                         &what);
+    if (cls == "none") {
+        std::printf("ok file\n");
+    } else {
+        std::printf("threw %s: %s\n", cls.c_str(), what.c_str());
+    }
+    return 0;
+}
+
 int compress(int argc, char** argv) {
     if (argc != 13) {
         std::fprintf(stderr, "compress needs 11 arguments\n");
@@ -231,8 +256,10 @@ int main(int argc, char** argv) {
             errors_device();
         } else if (mode == "compress") {
             return compress(argc, argv);
+        } else if (mode == "file") {
+            return file(argc, argv);
         } else {
-            std::fprintf(stderr, "usage: test_benes_api errors|errors-device|compress ...\n");
+            std::fprintf(stderr, "usage: test_benes_api errors|errors-device|file|compress ...\n");
             return 100;
         }
     } catch (const std::exception& e) {

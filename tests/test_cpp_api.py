@@ -65,3 +65,79 @@ def test_compress_golden_identity(tmp_path):
     run("compress", out, 3, 32, 0, 0, 4, 5, 0, "-", "-", "automatic")   # kind 3: identity
     with open(os.path.join(S.GOLDEN, "identity-32.benes"), "rb") as f:
         assert out.read_bytes() == f.read()
+
+
+def run_file(src, dst, width=4, g=5, placement=0):
+    out = run("file", src, dst, width, g, placement)
+    return out.strip().splitlines()[-1]
+
+
+def write_values(path, perm):
+    perm.astype(perm.dtype.newbyteorder("<"), copy=False).tofile(path)
+
+
+def test_compress_file_errors_before_the_device(tmp_path):
+    # The reference's file errors and their order (src/benes.cpp:498-553): the
+    # input file, its size, then compress; no output file is created when any
+    # of them fails.
+    import numpy as np
+
+    dst = tmp_path / "blob.bin"
+    assert run_file(tmp_path / "missing.bin", dst) == \
+        "threw runtime_error: cannot open permutation input file"
+    odd = tmp_path / "odd.bin"
+    odd.write_bytes(b"\0" * 6)
+    assert run_file(odd, dst) == \
+        "threw invalid_argument: permutation input has an invalid element width or size"
+    empty = tmp_path / "empty.bin"
+    empty.write_bytes(b"")
+    assert run_file(empty, dst).startswith("threw invalid_argument")
+    good = tmp_path / "good.bin"
+    write_values(good, np.arange(16, dtype=np.uint32))
+    assert run_file(good, dst, width=3).startswith("threw invalid_argument")
+    assert not dst.exists()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,width,g,pl", [(1, 4, 5, 0), (47, 8, 5, 2), (1000, 4, 3, 1),
+                                          (100003, 8, 4, 2), (300007, 4, 5, 0)])
+def test_compress_file_matches_oracle(tmp_path, n, width, g, pl):
+    import numpy as np
+
+    perm = np.random.default_rng(n).permutation(n).astype(np.uint32 if width == 4 else np.uint64)
+    src, dst = tmp_path / "perm.bin", tmp_path / "blob.bin"
+    write_values(src, perm)
+    assert run_file(src, dst, width, g, pl) == "ok file"
+    assert dst.read_bytes() == S.oracle_compress(perm.astype(np.uint64), g, pl)
+
+
+@pytest.mark.gpu
+def test_compress_file_errors_on_device(tmp_path):
+    import numpy as np
+
+    dst = tmp_path / "blob.bin"
+    dup = tmp_path / "dup.bin"
+    write_values(dup, np.array([0, 1, 1, 3], dtype=np.uint32))
+    assert run_file(dup, dst) == ("threw invalid_argument: input must contain every value from "
+                                  "zero through size minus one")
+    good = tmp_path / "good.bin"
+    write_values(good, np.arange(16, dtype=np.uint32))
+    assert run_file(good, dst, g=9) == \
+        "threw invalid_argument: middle_group_log2 must be between 1 and 5"
+    assert not dst.exists()
+    assert run_file(good, tmp_path / "no_such_dir" / "blob.bin") == \
+        "threw runtime_error: failed to write compressed permutation file"
+    dst.write_bytes(b"x" * 100000)   # an old, longer file is truncated
+    assert run_file(good, dst) == "ok file"
+    assert dst.read_bytes() == S.oracle_compress(np.arange(16, dtype=np.uint64))
+
+
+@pytest.mark.gpu
+def test_compress_file_reference_digest(tmp_path):
+    with open(os.path.join(S.GOLDEN, "fixtures.json")) as f:
+        rec = next(r for r in json.load(f) if r["name"] == "u64-fy-1060921")
+    perm = S.benes_hip.generate(rec["kind"], rec["n"], rec["seed"], rec["rate"], rec["width"])
+    src, dst = tmp_path / "perm.bin", tmp_path / "blob.bin"
+    write_values(src, perm)
+    assert run_file(src, dst, 8, rec["g"], rec["placement"]) == "ok file"
+    assert hashlib.sha256(dst.read_bytes()).hexdigest() == rec["sha256"]

@@ -160,3 +160,19 @@ def test_compress_file_reports_it_is_switched_off(tmp_path):
     with pytest.raises(RuntimeError, match="compress_file is not available"):
         B.compress_file(src, dst, 4)
     assert not dst.exists() and src.stat().st_size == 32
+
+
+def test_uniform_parallel_generator():
+    # A permutation for every size, the same for both widths, and (with several
+    # buckets, n >= 2^19) no position favoured for a given value.
+    for n in (1, 2, 3, 1000, (1 << 19) + 3, (1 << 20) + 1):
+        p = B.generate(B.UNIFORM_PARALLEL, n, 11, 0, 8)
+        assert np.array_equal(np.sort(p), np.arange(n, dtype=np.uint64)), n
+        assert np.array_equal(B.generate(B.UNIFORM_PARALLEL, n, 11, 0, 4).astype(np.uint64), p)
+    n = 1 << 20
+    quarter = [int(np.flatnonzero(B.generate(B.UNIFORM_PARALLEL, n, s, 0, 4) == 12345)[0]) >> 18
+               for s in range(400)]
+    assert np.bincount(quarter, minlength=4).min() > 60   # about 100 each
+    fixed = [int((B.generate(B.UNIFORM_PARALLEL, n, s, 0, 4) == np.arange(n)).sum())
+             for s in range(200)]
+    assert 0.6 < np.mean(fixed) < 1.5                      # Poisson(1) fixed points
