@@ -458,7 +458,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
     // Device copies of plans and item contexts.
     ws.plans.ensure(sizeof(Plan) * count);
     ws.items.ensure(sizeof(ItemCtx) * count);
-    ws.errs.ensure(sizeof(int) * count);
+    ws.errs.ensure(sizeof(int) * (count + 1));  // one flag per item, then the inverse overflow
     if (!out) ws.blob.ensure(total_words * 8);
     uint8_t* blob_ptr = out ? out : ws.blob.as<uint8_t>();
     std::vector<Plan> plans(count);
@@ -590,8 +590,35 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
     const uint64_t hook_bins = ceil_div(total_n, uint64_t{1} << kHookBinMinLog2);
     bool any_binned = false;
     for (const LevelPlan& lp : levels) any_binned = any_binned || lp.binned;
+    // The level-0 inverse is binned for batches of at least 2^31 entries (the
+    // test setting: from 2^20), in the smallest bins (from 2^18 entries) no
+    // item spans more than kInvBinSpan of.  Its staging is the union-find
+    // array, which then holds 8 bytes per entry.
+    const bool inv_binned =
+        max_df > 0 && (hook_binning == 1 ? total_n > (uint64_t{1} << 20)
+                       : hook_binning < 0 ? total_n >= (uint64_t{1} << 31) : false);
+    uint32_t inv_log2 = kHookBinMinLog2;
+    Launch inv_bin;
+    uint64_t inv_bins = 0, inv_scatter_blocks = 0;
+    if (inv_binned) {
+        for (const Job& j : jobs) {
+            while (((j.base + j.n - 1) >> inv_log2) - (j.base >> inv_log2) + 1 > kInvBinSpan) {
+                ++inv_log2;
+            }
+        }
+        SegList inv_list;
+        for (uint32_t i = 0; i < count; ++i) {
+            inv_list.add(i, ceil_div(jobs[i].n, kInvBinEntries), 0, 0, 0, 0, 0, false);
+        }
+        inv_bin = inv_list.commit(table);
+        inv_bins = ceil_div(total_n, uint64_t{1} << inv_log2);
+        inv_scatter_blocks = ceil_div(inv_bins, kXcds) * kXcds *
+                             ((uint64_t{1} << inv_log2) / kHookScatterEntries);
+    }
+    const uint64_t fill_words = (any_binned ? hook_bins * max_df : 0) + inv_bins;
     // One dispatch holds at most 2^32 - 1 work-items.
-    uint64_t most = std::max(subtree.blocks * sub.threads, describe_all.blocks * 64);
+    uint64_t most = std::max({subtree.blocks * sub.threads, describe_all.blocks * 64,
+                              inv_bin.blocks * kInvBinThreads, inv_scatter_blocks * kBlock});
     for (const LevelPlan& lp : levels) {
         most = std::max({most, lp.hbm.blocks * kBlock, lp.lds.blocks * kLevelThreads,
                          lp.pack.blocks * kBlock, lp.narrow.blocks * kNarrowThreads,
@@ -613,13 +640,13 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         }
         bool any_wide = false;
         for (const LevelPlan& lp : levels) any_wide = any_wide || lp.wide_uf;
-        ws.uf.ensure(total_n * (any_wide ? 8 : 4));
+        ws.uf.ensure(total_n * (any_wide || inv_binned ? 8 : 4));
         ws.desc.ensure(sizeof(NodeDesc) * std::max<uint64_t>(1, desc_base));
         ws.srows.ensure(8 * max_rows);
         ws.trows.ensure(8 * max_rows);
         ws.deferred.ensure(sizeof(DeferredEdge) * defer_cap);
         ws.deferred_counts.ensure(sizeof(unsigned long long) * 4 * max_df);
-        if (any_binned) ws.bin_fill.ensure(sizeof(uint32_t) * hook_bins * max_df);
+        if (fill_words) ws.bin_fill.ensure(sizeof(uint32_t) * fill_words);
         if (rulers) ws.rulers.ensure((sizeof(uint64_t) + sizeof(uint32_t)) * 2 * ruler_buckets);
     }
     BH_CHECK(hipMemcpyAsync(ws.plans.ptr, plans.data(), sizeof(Plan) * count,
@@ -634,12 +661,12 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         BH_LAUNCH(prof, "k_fill", k_fill_u32, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0,
                   st, static_cast<uint32_t*>(p), words, value);
     };
-    fill(ws.errs.ptr, count, 0);
+    fill(ws.errs.ptr, count + 1, 0);
     fill(blob_ptr, total_words * 2, 0);
     fill(ws.inv0.ptr, total_n * (wide_inverse ? 2 : 1), 0xFFFFFFFFu);
     if (max_df > 0) {
         fill(ws.deferred_counts.ptr, 2 * 4 * max_df, 0);
-        if (any_binned) fill(ws.bin_fill.ptr, hook_bins * max_df, 0);
+        if (fill_words) fill(ws.bin_fill.ptr, fill_words, 0);
     }
 
     const ItemCtx* ditems = ws.items.as<ItemCtx>();
@@ -654,9 +681,30 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
     auto validate = [&](auto* inv) {
         using VI = std::remove_pointer_t<decltype(inv)>;
         constexpr bool w64 = sizeof(VI) == 8;
-        BH_LAUNCH(prof, w64 ? "k_scatter_inverse_w64" : "k_scatter_inverse",
-                  (k_scatter_inverse<VI>), grid(per_entry), dim3(kBlock), 0, st, ditems,
-                  segp(per_entry), per_entry.count, inv, errs);
+        if (inv_binned) {
+            uint32_t* const inv_fill = ws.bin_fill.as<uint32_t>() + (fill_words - inv_bins);
+            // errs[count] is the overflow flag (a repeated value overfilled a bin).
+            int* const overflow = errs + count;
+            BH_LAUNCH(prof, w64 ? "k_inverse_bin_w64" : "k_inverse_bin", (k_inverse_bin<VI>),
+                      grid(inv_bin), dim3(kInvBinThreads), 0, st, ditems, segp(inv_bin),
+                      inv_bin.count, inv_log2, total_n, ws.uf.as<uint64_t>(), inv_fill, overflow,
+                      errs);
+            BH_LAUNCH(prof, w64 ? "k_inverse_scatter_w64" : "k_inverse_scatter",
+                      (k_inverse_scatter<VI>), dim3(static_cast<uint32_t>(inv_scatter_blocks)),
+                      dim3(kBlock), 0, st, inv, ws.uf.as<uint64_t>(), inv_fill, inv_bins,
+                      inv_log2, total_n);
+            // Return at once unless a bin overflowed.
+            const dim3 redo_grid(static_cast<uint32_t>(std::min<uint64_t>(per_entry.blocks, 2048)));
+            for (int pass = 0; pass < 2; ++pass) {
+                BH_LAUNCH(prof, "k_inverse_redo", (k_inverse_redo<VI>), redo_grid, dim3(kBlock), 0,
+                          st, ditems, segp(per_entry), per_entry.count, per_entry.blocks, inv,
+                          pass, overflow, errs);
+            }
+        } else {
+            BH_LAUNCH(prof, w64 ? "k_scatter_inverse_w64" : "k_scatter_inverse",
+                      (k_scatter_inverse<VI>), grid(per_entry), dim3(kBlock), 0, st, ditems,
+                      segp(per_entry), per_entry.count, inv, errs);
+        }
         BH_LAUNCH(prof, w64 ? "k_check_inverse_w64" : "k_check_inverse", (k_check_inverse<VI>),
                   grid(per_entry), dim3(kBlock), 0, st, ditems, segp(per_entry), per_entry.count,
                   inv, errs);

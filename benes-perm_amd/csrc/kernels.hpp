@@ -590,6 +590,167 @@ __global__ __launch_bounds__(kBlock) void k_hook_scatter(uint32_t* __restrict__ 
     }
 }
 
+// The level-0 inverse of large items, binned the same way.  Stored straight
+// (k_scatter_inverse), each random 4- or 8-byte store of inv[P[i]] = i
+// dirties a whole line that is written back long before its neighbours
+// arrive.  k_inverse_bin instead stages every store, packed as (value <<
+// bin_log2 | position within its bin of 2^bin_log2 entries; values below
+// 2^33 and bins of at most 2^31 entries fit), in the bin it falls in: a
+// block sorts its entries' stores by bin in LDS, reserves a run per bin with
+// one atomic and writes the runs out contiguously.  k_inverse_scatter then
+// stores the bins, each by the blocks of one XCD, so that a bin's inverse
+// words are written while they sit in that XCD's L2.  A permutation puts
+// exactly one store at each position, so bin b's run fits in positions
+// [b << bin_log2, (b + 1) << bin_log2) of the staging array (the union-find
+// words, unused until depth 0's first hooks); a store that would overflow its
+// bin can only come from a repeated value, but bins are shared by
+// neighbouring items of a batch, so it cannot tell whose: it raises
+// `overflow` instead, and k_inverse_redo then redoes the whole inverse
+// directly (returning at once when `overflow` is clear), so that
+// k_check_inverse blames exactly the items with repeated values.
+constexpr uint32_t kInvBinThreads = 1024;
+constexpr uint32_t kInvBinPerThread = 8;
+constexpr uint32_t kInvBinEntries = kInvBinThreads * kInvBinPerThread;
+constexpr uint32_t kInvBinSpan = 512;  // bins an item may touch, a multiple of 64
+
+template <typename VI>
+__global__ __launch_bounds__(kInvBinThreads) void k_inverse_bin(
+    const ItemCtx* __restrict__ items, const Seg* __restrict__ segs, uint32_t nsegs,
+    uint32_t bin_log2, uint64_t total_n, uint64_t* __restrict__ staged,
+    uint32_t* __restrict__ bin_fill, int* __restrict__ overflow, int* __restrict__ errs) {
+    __shared__ uint32_t fill[kInvBinSpan];
+    __shared__ uint32_t start[kInvBinSpan];
+    __shared__ uint32_t base[kInvBinSpan];
+    __shared__ uint64_t stage[kInvBinEntries];
+    __shared__ uint32_t total;
+    const Seg& sg = find_seg(segs, nsegs);
+    const ItemCtx it = items[sg.item];
+    const uint64_t bin0 = it.base >> bin_log2;
+    for (uint32_t i = threadIdx.x; i < kInvBinSpan; i += kInvBinThreads) fill[i] = 0;
+    __syncthreads();
+    const uint64_t i0 = (blockIdx.x - sg.block0) * uint64_t{kInvBinEntries};
+    uint64_t word[kInvBinPerThread];
+    uint32_t rank[kInvBinPerThread], lbin[kInvBinPerThread];
+    bool has[kInvBinPerThread];
+    bool bad = false;
+#pragma unroll
+    for (uint32_t k = 0; k < kInvBinPerThread; ++k) {
+        const uint64_t i = i0 + k * kInvBinThreads + threadIdx.x;
+        has[k] = false;
+        if (i >= it.n) continue;
+        const uint64_t v = load_src(it, i);
+        if (v >= it.n) {
+            bad = true;
+            continue;
+        }
+        const uint64_t pos = it.base + v;
+        has[k] = true;
+        lbin[k] = static_cast<uint32_t>((pos >> bin_log2) - bin0);
+        word[k] = (static_cast<uint64_t>(i) << bin_log2) | (pos & ((uint64_t{1} << bin_log2) - 1));
+        rank[k] = atomicAdd(&fill[lbin[k]], 1u);
+    }
+    if (bad) errs[sg.item] = 1;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t carry = 0;
+        for (uint32_t c = 0; c < kInvBinSpan; c += 64) {
+            const uint32_t b = c + lane_id();
+            const uint32_t f = fill[b];
+            const uint32_t inc = static_cast<uint32_t>(wave_inclusive_sum(f));
+            start[b] = carry + inc - f;
+            if (f) base[b] = atomicAdd(bin_fill + bin0 + b, f);
+            carry += __shfl(inc, 63);
+        }
+        if (lane_id() == 0) total = carry;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kInvBinPerThread; ++k) {
+        if (has[k]) stage[start[lbin[k]] + rank[k]] = word[k];
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < total; e += kInvBinThreads) {
+        const uint64_t w = stage[e];
+        // The bin of staged entry e: the last bin whose run starts at or
+        // before e (never an empty one: its successor would start there too,
+        // and the last bin's run ends at `total`, past e).
+        uint32_t lo = 0, hi = kInvBinSpan - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint64_t bin = bin0 + lo;
+        const uint64_t at = base[lo] + (e - start[lo]);
+        const uint64_t first = bin << bin_log2;
+        const uint64_t cap = min(uint64_t{1} << bin_log2, total_n - first);
+        if (at >= cap) {
+            *overflow = 1;  // more stores than positions: a repeated value somewhere
+            continue;
+        }
+        __builtin_nontemporal_store(w, staged + first + at);
+    }
+}
+
+// Bin b's inverse stores, by the blocks of one XCD (assignment as in
+// k_hook_scatter).
+template <typename VI>
+__global__ __launch_bounds__(kBlock) void k_inverse_scatter(VI* __restrict__ inv,
+                                                            const uint64_t* __restrict__ staged,
+                                                            const uint32_t* __restrict__ bin_fill,
+                                                            uint64_t bins, uint32_t bin_log2,
+                                                            uint64_t total_n) {
+    const uint32_t chunks = (uint32_t{1} << bin_log2) / kHookScatterEntries;
+    const uint32_t xcd = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+    const uint32_t round = local / chunks;
+    const uint64_t bin = uint64_t{round} * kXcds + (xcd + round) % kXcds;
+    const uint64_t chunk = local % chunks;
+    if (bin >= bins) return;
+    const uint64_t first = bin << bin_log2;
+    const uint64_t mask = (uint64_t{1} << bin_log2) - 1;
+    // Runs past the bin's positions were dropped (and flagged) when staged.
+    const uint64_t n = min(static_cast<uint64_t>(bin_fill[bin]),
+                           min(uint64_t{1} << bin_log2, total_n - first));
+#pragma unroll
+    for (uint32_t k = 0; k < kHookScatterPerThread; ++k) {
+        const uint64_t e = (chunk * kHookScatterPerThread + k) * kBlock + threadIdx.x;
+        if (e < n) {
+            const uint64_t w = __builtin_nontemporal_load(staged + first + e);
+            inv[first + (w & mask)] = static_cast<VI>(w >> bin_log2);
+        }
+    }
+}
+
+// The direct inverse after an overflow of the binned one (nothing when
+// *overflow is clear): every item's inverse reset to all-ones, then stored
+// directly, as k_scatter_inverse does.  A small grid strides over the
+// per-entry launch's virtual blocks, so a clear flag costs one short launch.
+template <typename VI>
+__global__ __launch_bounds__(kBlock) void k_inverse_redo(const ItemCtx* __restrict__ items,
+                                                         const Seg* __restrict__ segs,
+                                                         uint32_t nsegs, uint64_t vblocks,
+                                                         VI* __restrict__ inv, int pass,
+                                                         const int* __restrict__ overflow,
+                                                         int* __restrict__ errs) {
+    if (*overflow == 0) return;
+    for (uint64_t b = blockIdx.x; b < vblocks; b += gridDim.x) {
+        const Seg& sg = find_seg_at(segs, nsegs, b);
+        const ItemCtx it = items[sg.item];
+        for (uint64_t i = (b - sg.block0) * kBlock + threadIdx.x; i < it.n; i += sg.blocks * kBlock) {
+            if (pass == 0) {
+                inv[it.base + i] = static_cast<VI>(~VI{0});
+                continue;
+            }
+            const uint64_t v = load_src(it, i);
+            if (v >= it.n) {
+                errs[sg.item] = 1;
+                break;
+            }
+            inv[it.base + v] = static_cast<VI>(i);
+        }
+    }
+}
+
 // Rem walks longer than this many steps are handed to k_flatten.  A walk
 // follows one dependent load per step; on random
 // permutations the longest walk of a level is a few dozen steps, but the

@@ -37,9 +37,9 @@ def binned_kernels_ran(request):
     yield
     stats = B.kernel_stats()
     B.set_profiling(False)
-    if request.node.name.startswith("test_binning_modes"):
+    if request.node.name.startswith(("test_binning_modes", "test_binned_inverse_bad")):
         return
-    for kernel in ("k_hook_bin", "k_hook_scatter"):
+    for kernel in ("k_hook_bin", "k_hook_scatter", "k_inverse_bin", "k_inverse_scatter"):
         assert stats.get(kernel, (0, 0))[1] > 0, f"{kernel} did not run: {sorted(stats)}"
 
 
@@ -94,3 +94,19 @@ def test_binned_reference_digests_2e26():
 def test_binning_modes_are_checked():
     with pytest.raises(ValueError):
         B.set_hook_binning(2)
+
+
+def test_binned_inverse_bad_item_fails_alone():
+    # A repeated value overfills a bin of the binned inverse; the bin is shared
+    # with the neighbouring items, which must still come out right (the whole
+    # inverse is then redone directly and only the bad item is blamed).
+    good1 = np.random.default_rng(11).permutation((1 << 20) + 5).astype(np.uint32)
+    bad = np.zeros((1 << 20) + 3, np.uint32)              # every value 0
+    good2 = np.random.default_rng(12).permutation(1 << 20).astype(np.uint32)
+    bad2 = np.arange(1 << 19, dtype=np.uint32)
+    bad2[-1] = 0                                          # one repeat, no overflow
+    out = B.compress_batch([good1, bad, good2, bad2], [S.fmt(5, 0)] * 4)
+    assert out[0] == S.oracle_compress(good1.astype(np.uint64))
+    assert isinstance(out[1], ValueError) and "every value from zero" in str(out[1])
+    assert out[2] == S.oracle_compress(good2.astype(np.uint64))
+    assert isinstance(out[3], ValueError)
