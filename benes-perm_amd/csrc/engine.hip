@@ -132,7 +132,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     DevBuf input, level[2], ilevel[2], inv0, uf, srows, trows, desc, plans, items, segs, errs,
-        blob, deferred, deferred_counts, bin_fill, rulers;
+        blob, deferred, deferred_counts, bin_fill, rulers, lbits;
     std::vector<int> host_errs;
     PinnedRing ring;  // host <-> device staging of this workspace's stream
     Workspace() = default;
@@ -158,7 +158,7 @@ struct Workspace {
     void bind(Arena* a) {
         for (DevBuf* b : {&input, &level[0], &level[1], &ilevel[0], &ilevel[1], &inv0, &uf, &srows,
                           &trows, &desc, &plans, &items, &segs, &errs, &blob, &deferred,
-                          &deferred_counts, &bin_fill, &rulers}) {
+                          &deferred_counts, &bin_fill, &rulers, &lbits}) {
             b->release();
             b->arena = a;
         }
@@ -585,6 +585,9 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         subtree_list.add(i, uint64_t{1} << jobs[i].df, 0, 0, 0, 0, 0, false);
     }
     const Launch subtree = subtree_list.commit(table);
+    // Internal levels inside the largest subtree (rows kept per subtree).
+    uint32_t sub_levels = 1;
+    for (const Job& j : jobs) sub_levels = std::max(sub_levels, j.lay.plan.depth - j.df);
     // Per-depth bin counters, room for the smallest bins (only when a depth
     // bins its first hooks).
     const uint64_t hook_bins = ceil_div(total_n, uint64_t{1} << kHookBinMinLog2);
@@ -632,6 +635,8 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
     const uint64_t ruler_buckets = ceil_div(total_n, uint64_t{1} << kRulerLog2) + 1;
     // Scratch.
     ws.segs.ensure(sizeof(Seg) * std::max<size_t>(1, table.size()));
+    ws.lbits.ensure(sizeof(uint64_t) * subtree_row_words(sub.log2) * 2 * sub_levels *
+                    std::max<uint64_t>(1, subtree.blocks));
     ws.inv0.ensure(total_n * (wide_inverse ? 8 : 4));
     if (max_df > 0) {
         for (int i = 0; i < 2; ++i) {
@@ -880,16 +885,19 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
     }
     // An item with fusion depth df was last written at depth df - 1, into
     // level[(df - 1) & 1]; items with df == 0 read their own source.
-    auto subtree_kernel = [&](auto kernel) {
+    // Then the subtrees' switch rows (left in ws.lbits) are packed.
+    auto subtree_kernel = [&](auto kernel, auto emit) {
         BH_LAUNCH(prof, "k_subtree", kernel, grid(subtree), dim3(sub.threads), 0, st, ditems,
                   segp(subtree), subtree.count, ws.level[0].as<uint32_t>(),
-                  ws.level[1].as<uint32_t>(), blob, errs);
+                  ws.level[1].as<uint32_t>(), ws.lbits.as<uint64_t>(), sub_levels, blob, errs);
+        BH_LAUNCH(prof, "k_subtree_emit", emit, grid(subtree), dim3(kEmitThreads), 0, st, ditems,
+                  segp(subtree), subtree.count, ws.lbits.as<uint64_t>(), sub_levels, blob, errs);
     };
     switch (sub.log2 * 10000 + sub.threads) {
-        case 141024: subtree_kernel(k_subtree<14, 1024>); break;
-        case 131024: subtree_kernel(k_subtree<13, 1024>); break;
-        case 130512: subtree_kernel(k_subtree<13, 512>); break;
-        default: subtree_kernel(k_subtree<12, 512>); break;
+        case 141024: subtree_kernel(k_subtree<14, 1024>, k_subtree_emit<14>); break;
+        case 131024: subtree_kernel(k_subtree<13, 1024>, k_subtree_emit<13>); break;
+        case 130512: subtree_kernel(k_subtree<13, 512>, k_subtree_emit<13>); break;
+        default: subtree_kernel(k_subtree<12, 512>, k_subtree_emit<12>); break;
     }
     BH_CHECK(hipGetLastError());
 }
@@ -1958,7 +1966,7 @@ void benes_hip_release_workspace(void) {
     for (DevBuf* b : {&ws.input, &ws.level[0], &ws.level[1], &ws.ilevel[0], &ws.ilevel[1],
                       &ws.inv0, &ws.uf, &ws.srows, &ws.trows, &ws.desc, &ws.plans, &ws.items,
                       &ws.segs, &ws.errs, &ws.blob, &ws.deferred, &ws.deferred_counts,
-                      &ws.bin_fill, &ws.rulers}) {
+                      &ws.bin_fill, &ws.rulers, &ws.lbits}) {
         b->release();
     }
 }

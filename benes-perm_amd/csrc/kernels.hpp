@@ -1831,21 +1831,34 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
     }
 }
 
+// The subtree's switch rows of each level (both sides) leave the colouring
+// kernel through HBM and are packed by k_subtree_emit: the packing's address
+// arithmetic needs about twice the registers of the colouring, and kept in
+// one kernel it held the colouring to 4 waves per SIMD.  Rows of subtree b at
+// relative level k, side s: words [((b * levels + k) * 2 + s) * W, + W) of
+// `lbits`, W = subtree_row_words(SubLog2).
+__host__ __device__ constexpr uint32_t subtree_row_words(uint32_t sub_log2) {
+    return (1u << sub_log2) / 64 + 2;
+}
+
 template <uint32_t SubLog2, uint32_t Threads>
 __global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__ items,
                                                          const Seg* __restrict__ segs,
                                                          uint32_t nsegs,
                                                          const uint32_t* __restrict__ lev0,
                                                          const uint32_t* __restrict__ lev1,
+                                                         uint64_t* __restrict__ lbits,
+                                                         uint32_t levels,
                                                          uint64_t* __restrict__ blob_all,
                                                          const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
     constexpr uint32_t kSub = 1u << SubLog2;
+    constexpr uint32_t kRowWords = subtree_row_words(SubLog2);
     __shared__ uint16_t buf[2][kSub];
     __shared__ uint32_t uf[kSub];
-    __shared__ uint64_t sbits[kSub / 64 + 2];
-    __shared__ uint64_t tbits[kSub / 64 + 2];
+    __shared__ uint64_t sbits[kRowWords];
+    __shared__ uint64_t tbits[kRowWords];
     __shared__ uint16_t noff[kSub / 2 + 1];
     __shared__ SubtreeRoot root;
     const ItemCtx it = items[sg.item];
@@ -1858,22 +1871,10 @@ __global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__
     const uint64_t base = node_offset(P.n, d0, b0);
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nthreads = blockDim.x, nwaves = blockDim.x >> 6;
-    // Walks to the subtree root, and the address of the first node of every
-    // internal level on both sides, computed by different lanes at once:
-    // each is a chain of dependent table reads, and the emission of a level
-    // would otherwise wait for one.
-    __shared__ SideAddr first_addr[2][kSubLog2Max + 1];
+    // Walks to the subtree root (for the leaves' rank addresses).
     if (tid == 0) root.walk[0] = walk_to(P, 0, d0, b0);
     if (tid == 1) root.walk[1] = walk_to(P, 1, d0, b0);
     if (tid == 2) root.middle = middle_from(P, 0, 0, d0, b0);
-    if (tid >= 64 && tid < 64 + 2 * (kSubLog2Max + 1)) {
-        const uint32_t t = tid - 64, side = t & 1, k = t >> 1;
-        if (((n0 + (1u << k) - 1) >> k) > L) {
-            const uint32_t d = d0 + k;
-            const uint64_t path = b0 << k;
-            first_addr[side][k] = address_at(P, walk_to(P, side, d, path), d, path);
-        }
-    }
     // Depth d0 - 1 wrote level (d0 - 1) & 1; depth 0 reads the item's source.
     const uint32_t* plev = ((d0 - 1) & 1) ? lev1 : lev0;
     for (uint32_t i = tid; i < n0; i += nthreads) {
@@ -2037,6 +2038,12 @@ __global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__
             or_ballot(tbits, __ballot(t != 0), pi, qq);
         }
         __syncthreads();
+        // The level's rows, for k_subtree_emit.
+        uint64_t* const out = lbits + (static_cast<uint64_t>(blockIdx.x) * levels + k) * 2 * kRowWords;
+        for (uint32_t i = tid; i < kRowWords; i += nthreads) {
+            out[i] = sbits[i];
+            out[kRowWords + i] = tbits[i];
+        }
         for (uint32_t g = tid; g < total; g += nthreads) {
             const uint32_t pi = g >> s_log2, p = g & (S - 1);
             const uint32_t c = sub_count(n0, k, pi);
@@ -2053,9 +2060,70 @@ __global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__
             }
         }
         __syncthreads();
+        cur ^= 1;
+    }
+}
+
+// Packs the switch bits k_subtree left in `lbits`, level by level, with the
+// subtree's node offsets recomputed as there.
+constexpr uint32_t kEmitThreads = 256;
+
+template <uint32_t SubLog2>
+__global__ __launch_bounds__(kEmitThreads) void k_subtree_emit(const ItemCtx* __restrict__ items,
+                                                               const Seg* __restrict__ segs,
+                                                               uint32_t nsegs,
+                                                               const uint64_t* __restrict__ lbits,
+                                                               uint32_t levels,
+                                                               uint64_t* __restrict__ blob_all,
+                                                               const int* __restrict__ errs) {
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    constexpr uint32_t kSub = 1u << SubLog2;
+    constexpr uint32_t kRowWords = subtree_row_words(SubLog2);
+    __shared__ uint64_t sbits[kRowWords];
+    __shared__ uint64_t tbits[kRowWords];
+    __shared__ uint16_t noff[kSub / 2 + 1];
+    __shared__ SubtreeRoot root;
+    __shared__ SideAddr first_addr[2][kSubLog2Max + 1];
+    const ItemCtx it = items[sg.item];
+    const Plan& P = *it.plan;
+    uint64_t* blob = blob_all + it.blob;
+    const uint32_t L = P.leaf;
+    const uint32_t d0 = it.df;
+    const uint64_t b0 = blockIdx.x - sg.block0;
+    const uint32_t n0 = static_cast<uint32_t>(node_count(P.n, d0, b0));
+    const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
+    // Walks to the subtree root, and the address of the first node of every
+    // internal level on both sides, computed by different lanes at once:
+    // each is a chain of dependent table reads, and the emission of a level
+    // would otherwise wait for one.
+    if (tid == 0) root.walk[0] = walk_to(P, 0, d0, b0);
+    if (tid == 1) root.walk[1] = walk_to(P, 1, d0, b0);
+    if (tid >= 64 && tid < 64 + 2 * (kSubLog2Max + 1)) {
+        const uint32_t t = tid - 64, side = t & 1, k = t >> 1;
+        if (((n0 + (1u << k) - 1) >> k) > L) {
+            const uint32_t d = d0 + k;
+            const uint64_t path = b0 << k;
+            first_addr[side][k] = address_at(P, walk_to(P, side, d, path), d, path);
+        }
+    }
+    for (uint32_t k = 0;; ++k) {
+        const uint32_t nodes = 1u << k;
+        const uint32_t q = n0 >> k, m = n0 & (nodes - 1);
+        const uint32_t cmax = q + (m != 0 ? 1u : 0u);
+        if (cmax <= L) break;
+        for (uint32_t pi = tid; pi < nodes; pi += nthreads) {
+            noff[pi] = static_cast<uint16_t>(pi * q + large_before32(pi, m, k));
+        }
+        const uint64_t* const in =
+            lbits + (static_cast<uint64_t>(blockIdx.x) * levels + k) * 2 * kRowWords;
+        for (uint32_t i = tid; i < kRowWords; i += nthreads) {
+            sbits[i] = in[i];
+            tbits[i] = in[kRowWords + i];
+        }
+        __syncthreads();
         emit_level_runs(P, root, first_addr, d0, b0, n0, k, sbits, tbits, noff, blob);
         __syncthreads();
-        cur ^= 1;
     }
 }
 
