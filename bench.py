@@ -331,7 +331,7 @@ def random_format(rng, n: int) -> "B.FormatOptions":
     return B.FormatOptions(g, split(), split(), int(rng.integers(0, 3)))
 
 
-def served(count: int, rate_rps: float, seed: int = 5005, lanes: int = 3, max_batch: int = 64):
+def served(count: int, rate_rps: float, seed: int = 5005, lanes: int = 4, max_batch: int = 512):
     """Requests of log-uniform sizes in [2^8, 2^24], 80% u32 / 20% u64, the
     batched structure mix, random valid cluster splits and placements; host
     arrays in, host blobs out.  Poisson arrivals at `rate_rps` into the
@@ -401,6 +401,19 @@ def served(count: int, rate_rps: float, seed: int = 5005, lanes: int = 3, max_ba
         wall_b, lats_b, same_b, st_b = run(srv, None)
     lp = sorted(lat_plain)
     q = lambda xs, f: xs[min(len(xs) - 1, int(f * len(xs)))] * 1e3
+    # Plain calls per size class (log2 of the entries, in steps of 4): the
+    # largest class holds most entries, and its plain rate bounds what any
+    # batching can gain on this mix.
+    classes = {}
+    for (p, _), lat in zip(reqs, lat_plain):
+        c = min(20, int(np.log2(p.size)) // 4 * 4)
+        a = classes.setdefault(f"2^{c}..2^{c + 4}", {"requests": 0, "entries": 0, "seconds": 0.0})
+        a["requests"] += 1
+        a["entries"] += p.size
+        a["seconds"] += lat
+    for a in classes.values():
+        a["Mentries_per_s"] = a["entries"] / a["seconds"] / 1e6
+        a["share_of_entries"] = a["entries"] / total
     return {"workload": f"{count} requests, log-uniform 2^8..2^24, 80% u32 / 20% u64, "
                         "60/20/20 structures, random cluster splits and placements",
             "requests": count, "entries": total, "seed": seed, "lanes": lanes,
@@ -414,7 +427,8 @@ def served(count: int, rate_rps: float, seed: int = 5005, lanes: int = 3, max_ba
                       "p50_ms": q(lats_b, 0.5), "p99_ms": q(lats_b, 0.99),
                       "batches": st_b["batches"]},
             "plain": {"wall_s": t_plain, "Mentries_per_s": total / t_plain / 1e6,
-                      "p50_ms": q(lp, 0.5), "p99_ms": q(lp, 0.99)},
+                      "p50_ms": q(lp, 0.5), "p99_ms": q(lp, 0.99),
+                      "by_size_class": dict(sorted(classes.items()))},
             "burst_speedup_vs_plain": t_plain / wall_b,
             "each_result_equals_plain": same_p and same_b}
 

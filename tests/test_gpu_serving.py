@@ -93,3 +93,26 @@ def test_budget_and_bad_requests():
         assert blob.tobytes() == S.oracle_compress(B.generate(B.FISHER_YATES, 5000, 2))
         with pytest.raises(ValueError):
             srv.submit(np.zeros(0, np.uint32))
+
+
+def test_large_requests_overlap_under_a_64gb_budget():
+    # Requests up to 2^24 entries, uint64 among them, under the served
+    # workload's 64 GB budget: lanes run batches at the same time, and every
+    # blob is the request's own plain compress.
+    rng = np.random.default_rng(24)
+    reqs = []
+    for i in range(24):
+        n = int(2 ** rng.uniform(16, 24)) if i % 6 else (1 << 24) - int(rng.integers(0, 257))
+        width = 8 if i % 4 == 1 or i == 12 else 4
+        kind = (B.FISHER_YATES, B.SINGLE_CYCLE, B.NEAR_IDENTITY)[i % 3]
+        reqs.append((B.generate(kind, n, 24000 + i, 5000, width), S.fmt(5, i % 3)))
+    assert any(p.size >= (1 << 24) - 256 and p.dtype == np.uint64 for p, _ in reqs)
+    with B.Server(budget_bytes=64 << 30, max_batch=64, lanes=4) as srv:
+        ids = [srv.submit(p, f) for p, f in reqs]
+        results = [srv.wait(i) for i in ids]
+        stats = srv.stats()
+    for (perm, fmt), (blob, _) in zip(reqs, results):
+        assert not isinstance(blob, Exception), blob
+        assert blob.tobytes() == B.compress(perm, fmt)
+    assert stats["max_concurrent_batches"] > 1
+    assert stats["peak_bytes"] <= stats["budget_bytes"]
