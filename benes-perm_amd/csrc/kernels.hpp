@@ -1841,8 +1841,26 @@ __host__ __device__ constexpr uint32_t subtree_row_words(uint32_t sub_log2) {
     return (1u << sub_log2) / 64 + 2;
 }
 
+// Waves per SIMD the colouring kernel is compiled for.  2^12 x 512 (38 KB of
+// LDS, four workgroups per CU) and 2^13 x 1024 (76 KB, two per CU) fill all
+// 8 wave slots of a SIMD when the kernel keeps to 64 VGPRs; left to itself
+// the compiler takes 71-73 and the CU runs 7 or 6 waves per SIMD (2^12: one
+// workgroup fewer per CU).  The other shapes are held to 4 by their LDS.
+__host__ __device__ constexpr uint32_t subtree_waves(uint32_t sub_log2, uint32_t threads) {
+    return (sub_log2 == 12 && threads == 512) || (sub_log2 == 13 && threads == 1024) ? 8u : 4u;
+}
+
+// Unions each lane keeps in flight in the colouring kernel: four where
+// registers allow, fewer where the 64-VGPR budget above applies (3 and 2 are
+// the most that fit it without spills; below 8 waves the extra unions in
+// flight hid less than the extra waves do).
+__host__ __device__ constexpr uint32_t subtree_union_batch(uint32_t sub_log2, uint32_t threads) {
+    return subtree_waves(sub_log2, threads) < 8 ? 4u : (sub_log2 == 12 ? 3u : 2u);
+}
+
 template <uint32_t SubLog2, uint32_t Threads>
-__global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__ items,
+__global__ __launch_bounds__(Threads) __attribute__((amdgpu_waves_per_eu(
+    subtree_waves(SubLog2, Threads)))) void k_subtree(const ItemCtx* __restrict__ items,
                                                          const Seg* __restrict__ segs,
                                                          uint32_t nsegs,
                                                          const uint32_t* __restrict__ lev0,
@@ -1945,7 +1963,7 @@ __global__ __launch_bounds__(Threads) void k_subtree(const ItemCtx* __restrict__
         // input-slot forest as the HBM levels).  Each lane takes kUnionBatch
         // pairs at a time and issues their first hooks together (most settle
         // there), so the LDS round trips of independent unions overlap.
-        constexpr uint32_t kUnionBatch = 4;
+        constexpr uint32_t kUnionBatch = subtree_union_batch(SubLog2, Threads);
         for (uint32_t g0 = tid; g0 < total; g0 += kUnionBatch * nthreads) {
             uint32_t base[kUnionBatch], hi[kUnionBatch], lo[kUnionBatch], found[kUnionBatch];
             bool act[kUnionBatch];
