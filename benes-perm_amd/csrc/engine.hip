@@ -1450,21 +1450,25 @@ int compress_file(const char* input_path, const char* output_path, uint32_t widt
 // and are left out (`index` maps jobs back to items).
 void prepare_batch(benes_hip_item* items, uint64_t count, std::vector<Job>& jobs,
                    std::vector<uint64_t>& index) {
+    // A job holds its 8 KB plan: each is built in place, and the vector is
+    // sized up front so that growing it copies none.
+    jobs.reserve(jobs.size() + count);
+    index.reserve(index.size() + count);
     for (uint64_t i = 0; i < count; ++i) {
         Status st;
-        Job job;
+        Job& job = jobs.emplace_back();
         items[i].out_size = 0;
         items[i].status = BENES_HIP_OK;
         items[i].message[0] = 0;
         if (!prepare_job(items[i].perm, items[i].n, items[i].width_bytes, &items[i].format, job,
                          st)) {
+            jobs.pop_back();
             items[i].status = st.code;
             std::strncpy(items[i].message, st.message.c_str(), sizeof(items[i].message) - 1);
             items[i].message[sizeof(items[i].message) - 1] = 0;
             continue;
         }
         items[i].out_size = job.lay.plan.end_offset;
-        jobs.push_back(std::move(job));
         index.push_back(i);
     }
 }

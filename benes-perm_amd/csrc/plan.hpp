@@ -11,6 +11,8 @@
 #include "format.hpp"
 #include "status.hpp"
 
+#include <algorithm>
+#include <array>
 #include <cstring>
 #include <vector>
 
@@ -107,30 +109,37 @@ inline bool build_layout(uint64_t n, const FormatArgs& fmt, Layout& out, Status&
     p.depth = D;
     p.placement = placement;
 
-    // Bottom-up metrics for the two counts present at each depth.
+    // Bottom-up metrics for the two counts present at each depth.  Below
+    // depth D + 1 every count is at most a leaf (nodes at depth D hold at
+    // most L entries, and q(D + 1) + 1 <= L / 2 + 1), so the recursion starts
+    // there, and the switch bits of the top k levels of a depth-d subtree are
+    // needed for k <= D - d only (every lookup below stays within the routing
+    // depth, and so does the recursion from depth d + 1).  A table over all
+    // kMaxDepth depths and levels cost about 20 us per call, most of a small
+    // batch item's host time.
     constexpr int K = kMaxDepth;
     const uint64_t L = p.leaf;
-    std::vector<Sat> S((K + 1) * 2), M((K + 1) * 2);
-    std::vector<Sat> PSB((K + 1) * 2 * (K + 1));  // [d][x][k]
+    const int top = std::min(K, static_cast<int>(D) + 1);  // deepest depth recursed from
+    const int KK = top;                                    // levels kept in the table
+    std::array<Sat, (K + 1) * 2> S{}, M{};
+    std::vector<Sat> PSB(static_cast<size_t>(top + 1) * 2 * (KK + 1));  // [d][x][k], zeroed
     auto q = [&](int d) { return d >= 64 ? uint64_t{0} : (n >> d); };
     auto idx = [&](int d, uint64_t c) { return static_cast<int>(c - q(d)); };
-    auto psb = [&](int d, int x, int k) -> Sat& { return PSB[(d * 2 + x) * (K + 1) + k]; };
+    auto psb = [&](int d, int x, int k) -> Sat& { return PSB[(d * 2 + x) * (KK + 1) + k]; };
     for (int d = K; d >= 0; --d) {
         for (int x = 0; x < 2; ++x) {
             const uint64_t c = q(d) + x;
             Sat& s = S[d * 2 + x];
             Sat& m = M[d * 2 + x];
-            if (c <= L || d == K) {
+            if (c <= L || d >= top) {
                 s = Sat{};
                 m = Sat{c <= 32 ? rank_bits(static_cast<uint32_t>(c)) : 0, c > L};
-                for (int k = 0; k <= K; ++k) psb(d, x, k) = Sat{};
                 continue;
             }
             const int a = idx(d + 1, c / 2 + c % 2), b = idx(d + 1, c / 2);
             s = sat_add(Sat{c / 2}, sat_add(S[(d + 1) * 2 + a], S[(d + 1) * 2 + b]));
             m = sat_add(M[(d + 1) * 2 + a], M[(d + 1) * 2 + b]);
-            psb(d, x, 0) = Sat{};
-            for (int k = 1; k <= K; ++k) {
+            for (int k = 1; k <= KK && k <= static_cast<int>(D) - d; ++k) {
                 psb(d, x, k) = sat_add(Sat{c / 2},
                                        sat_add(psb(d + 1, a, k - 1), psb(d + 1, b, k - 1)));
             }
