@@ -392,11 +392,17 @@ def served(count: int, rate_rps: float, seed: int = 5005, lanes: int = 4, max_ba
                    for (b, _), bp in zip(results, blobs_plain))
         return wall, lats, same, stats
 
-    # One long-lived server, as a service runs: a first untimed burst sizes
-    # its lanes' workspaces and staging and runs the tuner's trial calls,
-    # as the plain calls above had theirs before being timed.
+    # One long-lived server, as a service runs: untimed bursts size its
+    # lanes' workspaces and staging and run the tuner's trial calls, as the
+    # plain calls above had theirs before being timed.  A burst forms about
+    # 50 batches over a dozen size classes, some with fewer than the eight
+    # trial calls a class takes, so bursts repeat (at most four) until no
+    # class is still trying shapes.
     with B.Server(budget_bytes=64 << 30, max_batch=max_batch, lanes=lanes) as srv:
-        run(srv, None)
+        for _ in range(4):
+            run(srv, None)
+            if "(trials" not in B.autotune_report():
+                break
         wall_p, lats_p, same_p, st_p = run(srv, gaps)
         wall_b, lats_b, same_b, st_b = run(srv, None)
     lp = sorted(lat_plain)

@@ -129,24 +129,6 @@ __device__ __forceinline__ uint64_t wave_inclusive_sum(uint64_t v) {
     return v;
 }
 
-// ORs `len` bits, bit b given by get(b), into blob bits [dst, dst + len) from
-// one lane: 64 bits at a time, each with at most two atomics (neighbouring
-// runs of other lanes share the edge words).
-template <typename Get>
-__device__ __forceinline__ void lane_or_bits(uint64_t* blob, uint64_t dst, uint64_t len, Get get) {
-    for (uint64_t b0 = 0; b0 < len; b0 += 64) {
-        const uint32_t cnt = static_cast<uint32_t>(min(uint64_t{64}, len - b0));
-        uint64_t acc = 0;
-        for (uint32_t b = 0; b < cnt; ++b) acc |= static_cast<uint64_t>(get(b0 + b) ? 1 : 0) << b;
-        if (acc == 0) continue;
-        const uint64_t at = dst + b0;
-        auto* w = reinterpret_cast<unsigned long long*>(blob) + (at >> 6);
-        const uint32_t sh = at & 63;
-        atomicOr(w, static_cast<unsigned long long>(acc << sh));
-        if (sh && (acc >> (64 - sh))) atomicOr(w + 1, static_cast<unsigned long long>(acc >> (64 - sh)));
-    }
-}
-
 // ORs the low `len` (<= 64) bits of `v` into blob bits [dst, dst + len).
 __device__ __forceinline__ void or_word(uint64_t* blob, uint64_t dst, uint64_t v) {
     if (v == 0) return;
@@ -164,16 +146,6 @@ __device__ __forceinline__ uint64_t row_range(const uint64_t* rows, uint64_t row
     uint64_t v = rows[w] >> sh;
     if (sh && sh + len > 64) v |= rows[w + 1] << (64 - sh);
     return len >= 64 ? v : v & ((uint64_t{1} << len) - 1);
-}
-
-// ORs bits [first, first + len) of a bit row into blob bits [dst, dst + len)
-// from one lane, a word at a time.
-__device__ __forceinline__ void lane_or_row(uint64_t* blob, uint64_t dst, const uint64_t* rows,
-                                            uint64_t row, uint64_t first, uint64_t len) {
-    for (uint64_t b = 0; b < len; b += 64) {
-        const uint32_t cnt = static_cast<uint32_t>(min(uint64_t{64}, len - b));
-        or_word(blob, dst + b, row_range(rows, row, first + b, cnt));
-    }
 }
 
 __device__ __forceinline__ uint32_t row_bit(const uint64_t* rows, uint64_t row, uint64_t pair) {
@@ -1369,10 +1341,34 @@ __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ ite
         }
         return;
     }
-    for (uint64_t pi = wi; pi < nodes; pi += wpj) {
+    // The packet where node shares differ (or a group's last, partial
+    // packet): each node's share goes to its own address.  With fewer nodes
+    // than the job has threads (a tall cluster group near its root, where a
+    // share runs to 2^(g-1) bits), the threads split each share by output
+    // word; one thread per share would write up to 2^(g-7) words in a row
+    // while the rest of the launch waits.
+    const uint64_t tpn = nodes < wpj ? wpj / nodes : 1;  // threads per node
+    const uint64_t first_node = tpn > 1 ? wi / tpn : wi;
+    const uint64_t node_step = tpn > 1 ? nodes : wpj;    // tpn > 1: one node per thread
+    const uint64_t k0 = tpn > 1 ? wi % tpn : 0;
+    for (uint64_t pi = first_node; pi < nodes; pi += node_step) {
         const NodeDesc& nd = dsc[b0 + pi];
-        lane_or_row(blob, secbase + bit_of(nd.a[side], first_pair), rows, nd.row, first_pair,
-                    packet_share(nd.cnt, packet, span_log2, P.leaf));
+        const uint64_t len = packet_share(nd.cnt, packet, span_log2, P.leaf);
+        if (len == 0) continue;
+        const uint64_t dst = secbase + bit_of(nd.a[side], first_pair);
+        const uint64_t w_end = ((dst + len - 1) >> 6) + 1;
+        for (uint64_t ow = (dst >> 6) + k0; ow < w_end; ow += tpn) {
+            const uint64_t lo = max(ow << 6, dst), hi = min((ow + 1) << 6, dst + len);
+            const uint64_t v =
+                row_range(rows, nd.row, first_pair + (lo - dst), static_cast<uint32_t>(hi - lo))
+                << (lo & 63);
+            if (hi - lo == 64) {
+                blob[ow] = v;
+            } else if (v) {
+                atomicOr(reinterpret_cast<unsigned long long*>(blob + ow),
+                         static_cast<unsigned long long>(v));
+            }
+        }
     }
 }
 
@@ -1819,11 +1815,13 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
                     pi < end ? packet_share(sub_count(n0, k, pi), pk, span_log2, L) : 0;
                 const uint64_t incl = wave_inclusive_sum(share);
                 if (share) {
+                    // The lane's share, up to 64 row bits at a time.
                     const uint32_t pos0 = noff[pi] + first_pair;
-                    lane_or_bits(blob, run0 + carry + incl - share, share, [&](uint64_t b) {
-                        const uint32_t pos = pos0 + static_cast<uint32_t>(b);
-                        return ((rows[pos >> 6] >> (pos & 63)) & 1) != 0;
-                    });
+                    const uint64_t dst = run0 + carry + incl - share;
+                    for (uint32_t b = 0; b < share; b += 64) {
+                        const uint32_t cnt = min(64u, static_cast<uint32_t>(share) - b);
+                        or_word(blob, dst + b, lds_bits(rows, pos0 + b, cnt));
+                    }
                 }
                 carry += __shfl(incl, 63);
             }

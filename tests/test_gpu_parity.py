@@ -113,6 +113,21 @@ def test_global_levels(n):
             assert B.compress(perm, S.fmt(g, pl, ic, oc)) == S.oracle_compress(perm, g, pl, ic, oc)
 
 
+@pytest.mark.parametrize("n", [(1 << 19) - 3, (1 << 20) + 12345])
+def test_tall_cluster_groups(n):
+    # One cluster group spanning most levels: near its root a packet's share
+    # of one node runs to 2^(g-1) bits, and in the packet where node sizes
+    # differ (or the group's last, partial packet) each share is written at
+    # its node's own address by several threads, word by word.
+    rng = np.random.default_rng(0x7A11 + n)
+    for g, pl in ((4, 0), (5, 2), (3, 1)):
+        d = S.routing_depth(n, g)
+        for ic, oc in (((d,), (d,)), ((2, d - 4, 2), (d,)), ((1, d - 1), (d - 2, 2))):
+            perm = rng.permutation(n).astype(np.uint32)
+            assert B.compress(perm, S.fmt(g, pl, ic, oc)) == S.oracle_compress(perm, g, pl, ic, oc), \
+                (n, g, pl, ic, oc)
+
+
 @pytest.mark.parametrize("shape", B.SUBTREE_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
 def test_every_subtree_shape(shape):
     # Every subtree shape the tuner may pick gives the same bytes: sizes
