@@ -890,7 +890,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         BH_LAUNCH(prof, "k_subtree", kernel, grid(subtree), dim3(sub.threads), 0, st, ditems,
                   segp(subtree), subtree.count, ws.level[0].as<uint32_t>(),
                   ws.level[1].as<uint32_t>(), ws.lbits.as<uint64_t>(), sub_levels, blob, errs);
-        BH_LAUNCH(prof, "k_subtree_emit", emit, grid(subtree), dim3(kEmitThreads), 0, st, ditems,
+        BH_LAUNCH(prof, "k_subtree_emit", emit, grid(subtree), dim3(emit_threads(sub.log2)), 0, st, ditems,
                   segp(subtree), subtree.count, ws.lbits.as<uint64_t>(), sub_levels, blob, errs);
     };
     switch (sub.log2 * 10000 + sub.threads) {

@@ -2081,11 +2081,17 @@ __global__ __launch_bounds__(Threads) __attribute__((amdgpu_waves_per_eu(
 }
 
 // Packs the switch bits k_subtree left in `lbits`, level by level, with the
-// subtree's node offsets recomputed as there.
-constexpr uint32_t kEmitThreads = 256;
+// subtree's node offsets recomputed as there.  One workgroup per subtree, of
+// one wave per 2^6 entries of subtree capacity (2^12: 64 threads, 2^13: 128,
+// 2^14: 256): measured at 2^26, larger workgroups for the smaller subtrees
+// left lanes idle (2^12 x 256 threads: 0.89 ms per compress, x 128: 0.56,
+// x 64: 0.40) and smaller ones lengthened the larger subtrees' packing.
+__host__ __device__ constexpr uint32_t emit_threads(uint32_t sub_log2) {
+    return sub_log2 <= 12 ? 64u : (sub_log2 == 13 ? 128u : 256u);
+}
 
 template <uint32_t SubLog2>
-__global__ __launch_bounds__(kEmitThreads) void k_subtree_emit(const ItemCtx* __restrict__ items,
+__global__ __launch_bounds__(emit_threads(SubLog2)) void k_subtree_emit(const ItemCtx* __restrict__ items,
                                                                const Seg* __restrict__ segs,
                                                                uint32_t nsegs,
                                                                const uint64_t* __restrict__ lbits,
@@ -2113,10 +2119,14 @@ __global__ __launch_bounds__(kEmitThreads) void k_subtree_emit(const ItemCtx* __
     // internal level on both sides, computed by different lanes at once:
     // each is a chain of dependent table reads, and the emission of a level
     // would otherwise wait for one.
+    // (Lanes of a second wave where there is one, else lanes 2 and up.)
+    constexpr uint32_t kFirstAddrLane = emit_threads(SubLog2) >= 128 ? 64u : 2u;
+    static_assert(kFirstAddrLane + 2 * (kSubLog2Max + 1) <= emit_threads(SubLog2),
+                  "every first address needs a lane");
     if (tid == 0) root.walk[0] = walk_to(P, 0, d0, b0);
     if (tid == 1) root.walk[1] = walk_to(P, 1, d0, b0);
-    if (tid >= 64 && tid < 64 + 2 * (kSubLog2Max + 1)) {
-        const uint32_t t = tid - 64, side = t & 1, k = t >> 1;
+    if (tid >= kFirstAddrLane && tid < kFirstAddrLane + 2 * (kSubLog2Max + 1)) {
+        const uint32_t t = tid - kFirstAddrLane, side = t & 1, k = t >> 1;
         if (((n0 + (1u << k) - 1) >> k) > L) {
             const uint32_t d = d0 + k;
             const uint64_t path = b0 << k;
