@@ -1887,15 +1887,40 @@ __global__ __launch_bounds__(Threads) __attribute__((amdgpu_waves_per_eu(
     const uint64_t base = node_offset(P.n, d0, b0);
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nthreads = blockDim.x, nwaves = blockDim.x >> 6;
-    // Walks to the subtree root (for the leaves' rank addresses).
-    if (tid == 0) root.walk[0] = walk_to(P, 0, d0, b0);
-    if (tid == 1) root.walk[1] = walk_to(P, 1, d0, b0);
-    if (tid == 2) root.middle = middle_from(P, 0, 0, d0, b0);
+    // Walks to the subtree root (for the leaves' rank addresses).  Each is a
+    // chain of dependent table reads as long as the subtree is deep, which
+    // the first barrier waits for: the two sides run in two lanes of one
+    // branch and the middle-section walk in another wave, all at once (in
+    // three branches of one wave they took 11.6 of the 13 us before it).
+    if (tid < 2) root.walk[tid] = walk_to(P, static_cast<int>(tid), d0, b0);
+    if (tid == 64) root.middle = middle_from(P, 0, 0, d0, b0);
     // Depth d0 - 1 wrote level (d0 - 1) & 1; depth 0 reads the item's source.
+    // Every lane issues its loads before storing any (a loop that stored
+    // each value before loading the next paid one memory latency per entry).
     const uint32_t* plev = ((d0 - 1) & 1) ? lev1 : lev0;
-    for (uint32_t i = tid; i < n0; i += nthreads) {
-        buf[0][i] = static_cast<uint16_t>(d0 == 0 ? load_src(it, base + i)
-                                                  : plev[it.base + base + i]);
+    {
+        constexpr uint32_t kPer = kSub / Threads;
+        static_assert(kPer * Threads == kSub, "a subtree's entries split evenly over the lanes");
+        uint32_t v[kPer];
+        if (d0 == 0) {
+#pragma unroll
+            for (uint32_t j = 0; j < kPer; ++j) {
+                const uint32_t i = tid + j * Threads;
+                v[j] = i < n0 ? static_cast<uint32_t>(load_src(it, base + i)) : 0u;
+            }
+        } else {
+            const uint32_t* src = plev + it.base + base;
+#pragma unroll
+            for (uint32_t j = 0; j < kPer; ++j) {
+                const uint32_t i = tid + j * Threads;
+                v[j] = i < n0 ? __builtin_nontemporal_load(src + i) : 0u;
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+            const uint32_t i = tid + j * Threads;
+            if (i < n0) buf[0][i] = static_cast<uint16_t>(v[j]);
+        }
     }
     __syncthreads();
     const int rank_side = P.placement == 1 ? 0 : 1;
@@ -2123,14 +2148,21 @@ __global__ __launch_bounds__(emit_threads(SubLog2)) void k_subtree_emit(const It
     constexpr uint32_t kFirstAddrLane = emit_threads(SubLog2) >= 128 ? 64u : 2u;
     static_assert(kFirstAddrLane + 2 * (kSubLog2Max + 1) <= emit_threads(SubLog2),
                   "every first address needs a lane");
-    if (tid == 0) root.walk[0] = walk_to(P, 0, d0, b0);
-    if (tid == 1) root.walk[1] = walk_to(P, 1, d0, b0);
-    if (tid >= kFirstAddrLane && tid < kFirstAddrLane + 2 * (kSubLog2Max + 1)) {
-        const uint32_t t = tid - kFirstAddrLane, side = t & 1, k = t >> 1;
-        if (((n0 + (1u << k) - 1) >> k) > L) {
+    // The root walks (lanes 0 and 1) and the first-node addresses take one
+    // code path, so that lanes of one wave walk together rather than in turn.
+    if (tid < 2 || (tid >= kFirstAddrLane && tid < kFirstAddrLane + 2 * (kSubLog2Max + 1))) {
+        const bool is_root = tid < 2;
+        const uint32_t t = is_root ? 0u : tid - kFirstAddrLane;
+        const uint32_t side = is_root ? tid : (t & 1), k = is_root ? 0u : (t >> 1);
+        if (is_root || ((n0 + (1u << k) - 1) >> k) > L) {
             const uint32_t d = d0 + k;
             const uint64_t path = b0 << k;
-            first_addr[side][k] = address_at(P, walk_to(P, side, d, path), d, path);
+            const Walk w = walk_to(P, static_cast<int>(side), d, path);
+            if (is_root) {
+                root.walk[side] = w;
+            } else {
+                first_addr[side][k] = address_at(P, w, d, path);
+            }
         }
     }
     for (uint32_t k = 0;; ++k) {
