@@ -267,8 +267,8 @@ def compress_into(perm: np.ndarray, fmt: FormatOptions | None = None,
 def compress_file(input_path, output_path, width: int = 4,
                   fmt: FormatOptions | None = None) -> int:
     """The reference's compress_file: raw little-endian values of `width`
-    bytes from `input_path`, the blob written to `output_path` (streamed
-    through pinned memory both ways).  Returns the blob's size."""
+    bytes from `input_path`, the blob written to `output_path` (the file is
+    read whole and compressed as a host array).  Returns the blob's size."""
     f, keep = _format(fmt)
     size = ctypes.c_uint64()
     err = ctypes.create_string_buffer(256)

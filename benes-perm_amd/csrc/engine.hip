@@ -1369,14 +1369,13 @@ struct FdCloser {
     }
 };
 
-constexpr bool kCompressFileEnabled = false;
-
-// compress_file (src/benes.cpp:498-553): the file's raw values are read
-// straight into the pinned ring and on to HBM chunk by chunk, the blob goes
-// from HBM through the ring into the output file; no host copy of either is
-// made.  Checks and messages follow the reference in its order: the input
-// file, its size, the read, then compression, then the output (created or
-// truncated only once compression has succeeded).
+// compress_file (src/benes.cpp:498-553) over the host path of
+// benes_hip_compress: the file is read whole into host memory, compressed as
+// a host array of little-endian values (the staging ring takes it to HBM,
+// where it is validated), and the blob is written out.  Checks and messages
+// follow the reference in its order: the input file, its size, the read,
+// then compression (with compress's own error order), then the output,
+// created or truncated only once compression has succeeded.
 int compress_file(const char* input_path, const char* output_path, uint32_t width,
                   const benes_hip_format* format, uint64_t* out_size, double* device_seconds,
                   char* err, size_t err_len) {
@@ -1393,56 +1392,54 @@ int compress_file(const char* input_path, const char* output_path, uint32_t widt
         st.set(kInvalidArgument, "permutation input has an invalid element width or size");
         return finish(st, err, err_len);
     }
+    std::vector<uint8_t> raw;
+    std::vector<uint8_t> blob;
+    try {
+        raw.resize(length);
+    } catch (const std::bad_alloc&) {
+        st.set(kLength, "allocation failed");
+        return finish(st, err, err_len);
+    }
+    if (!pread_all(in.fd, raw.data(), length, 0)) {
+        st.set(kRuntime, "failed to read permutation input file");
+        return finish(st, err, err_len);
+    }
+    ::close(in.fd);
+    in.fd = -1;
     const uint64_t n = length / width;
-    with_workspace(st, [&](Workspace& ws) {
-        auto t0 = std::chrono::steady_clock::now();
-        ws.input.ensure(length);
-        if (!ws.ring.upload_stream(
-                ws.input.as<uint8_t>(), length,
-                [&](uint8_t* dst, uint64_t off, uint64_t len) {
-                    return pread_all(in.fd, dst, len, off);
-                },
-                ws.stream)) {
-            BH_CHECK(hipStreamSynchronize(ws.stream));
-            st.set(kRuntime, "failed to read permutation input file");
-            return;
-        }
-        note_host_time("host:file_read", t0);
-        std::vector<Job> jobs(1);
-        if (!prepare_job(ws.input.ptr, n, width, format, jobs[0], st)) {
-            BH_CHECK(hipStreamSynchronize(ws.stream));
-            return;
-        }
-        std::vector<int> flags;
-        run_batch(ws, jobs, nullptr, nullptr, flags, device_seconds);
-        if (flags[0]) {
-            st.set(kInvalidArgument, kNotPermutation);
-            return;
-        }
-        const uint64_t end = jobs[0].lay.plan.end_offset;
-        t0 = std::chrono::steady_clock::now();
-        FdCloser out{output_path
-                         ? ::open(output_path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0666)
-                         : -1};
-        bool ok = out.fd >= 0 &&
-                  ws.ring.download_stream(
-                      ws.blob.as<uint8_t>(), end,
-                      [&](const uint8_t* src, uint64_t off, uint64_t len) {
-                          return pwrite_all(out.fd, src, len, off);
-                      },
-                      ws.stream);
-        BH_CHECK(hipStreamSynchronize(ws.stream));
-        if (out.fd >= 0) {
-            ok = ::close(out.fd) == 0 && ok;
-            out.fd = -1;
-        }
-        note_host_time("host:file_write", t0);
-        if (!ok) {
-            st.set(kRuntime, "failed to write compressed permutation file");
-            return;
-        }
-        if (out_size) *out_size = end;
-    });
+    // A size query, then the compress into a host buffer of that size.
+    uint64_t size = 0;
+    int rc = compress_one(raw.data(), false, n, width, format, nullptr, nullptr, 0, &size,
+                          nullptr, nullptr, err, err_len);
+    if (rc != BENES_HIP_BUFFER_TOO_SMALL) {
+        if (rc != BENES_HIP_OK) return rc;
+        st.set(kLogic, "size query failed");
+        return finish(st, err, err_len);
+    }
+    try {
+        blob.resize(size);
+    } catch (const std::bad_alloc&) {
+        st.set(kLength, "allocation failed");
+        return finish(st, err, err_len);
+    }
+    rc = compress_one(raw.data(), false, n, width, format, blob.data(), nullptr, size, &size,
+                      nullptr, device_seconds, err, err_len);
+    if (rc != BENES_HIP_OK) return rc;
+    raw = std::vector<uint8_t>();
+    const auto t0 = std::chrono::steady_clock::now();
+    FdCloser out{output_path ? ::open(output_path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0666)
+                             : -1};
+    bool ok = out.fd >= 0 && pwrite_all(out.fd, blob.data(), size, 0);
+    if (out.fd >= 0) {
+        ok = ::close(out.fd) == 0 && ok;
+        out.fd = -1;
+    }
+    note_host_time("host:file_write", t0);
+    if (!ok) {
+        st.set(kRuntime, "failed to write compressed permutation file");
+        return finish(st, err, err_len);
+    }
+    if (out_size) *out_size = size;
     return finish(st, err, err_len);
 }
 
@@ -1747,17 +1744,8 @@ int benes_hip_compress_file(const char* input_path, const char* output_path,
                             uint32_t width_bytes, const benes_hip_format* format,
                             uint64_t* out_size, double* device_seconds, char* err,
                             size_t err_len) {
-    // The streaming path below is switched off: a run of it stopped making
-    // progress and its cause has not been found.  Callers get an error
-    // instead of a call that may not return.
-    if (kCompressFileEnabled) {
-        return compress_file(input_path, output_path, width_bytes, format, out_size,
-                             device_seconds, err, err_len);
-    }
-    Status st;
-    if (out_size) *out_size = 0;
-    st.set(kRuntime, "compress_file is not available in this build");
-    return finish(st, err, err_len);
+    return compress_file(input_path, output_path, width_bytes, format, out_size, device_seconds,
+                         err, err_len);
 }
 
 int benes_hip_compress_device(const void* d_perm, uint64_t n, uint32_t width_bytes,

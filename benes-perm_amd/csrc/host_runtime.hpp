@@ -220,52 +220,6 @@ public:
         }
     }
 
-    // Bytes [0, total) produced chunk by chunk by fill(dst, offset, len) ->
-    // device range at `dev_base`, queued on `s`: one chunk is filled while
-    // the other is in flight.  Returns false as soon as fill does.
-    template <typename Fill>
-    bool upload_stream(uint8_t* dev_base, uint64_t total, Fill&& fill, hipStream_t s) {
-        init();
-        for (uint64_t c0 = 0, k = 0; c0 < total; c0 += kChunk, ++k) {
-            const int b = static_cast<int>(k & 1);
-            const uint64_t len = std::min(kChunk, total - c0);
-            wait(b);
-            if (!fill(buf_[b], c0, len)) return false;
-            BH_CHECK(hipMemcpyAsync(dev_base + c0, buf_[b], len, hipMemcpyHostToDevice, s));
-            BH_CHECK(hipEventRecord(ev_[b], s));
-            pending_[b] = true;
-        }
-        return true;
-    }
-
-    // Device range [dev_base, dev_base + total), after the work already
-    // queued on `s`, handed to sink(src, offset, len) chunk by chunk in
-    // order while the next chunk is in flight.  Returns false as soon as
-    // sink does.
-    template <typename Sink>
-    bool download_stream(const uint8_t* dev_base, uint64_t total, Sink&& sink, hipStream_t s) {
-        if (total == 0) return true;
-        init();
-        const uint64_t chunks = (total + kChunk - 1) / kChunk;
-        auto issue = [&](uint64_t k) {
-            const int b = static_cast<int>(k & 1);
-            const uint64_t c0 = k * kChunk, c1 = std::min(total, c0 + kChunk);
-            wait(b);
-            BH_CHECK(hipMemcpyAsync(buf_[b], dev_base + c0, c1 - c0, hipMemcpyDeviceToHost, s));
-            BH_CHECK(hipEventRecord(ev_[b], s));
-            pending_[b] = true;
-        };
-        issue(0);
-        for (uint64_t k = 0; k < chunks; ++k) {
-            if (k + 1 < chunks) issue(k + 1);
-            const int b = static_cast<int>(k & 1);
-            const uint64_t c0 = k * kChunk, c1 = std::min(total, c0 + kChunk);
-            wait(b);
-            if (!sink(buf_[b], c0, c1 - c0)) return false;
-        }
-        return true;
-    }
-
 private:
     void init() {
         if (buf_[0]) return;

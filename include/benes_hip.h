@@ -177,18 +177,15 @@ void benes_hip_server_destroy(benes_hip_server* server);
 /* ---- Files: ingest and egress. ----
  * Replaces compress_file (src/benes.cpp:498-553): raw little-endian values of
  * width_bytes (4 or 8) from `input_path`, the blob written to `output_path`.
- * The input is read chunk by chunk straight into pinned memory and on to HBM
- * (validated there), and the blob streams from HBM into the output file; no
- * host copy of either is made.  Errors and messages are the reference's, in
+ * The file is read whole into host memory and compressed as a host array
+ * (as benes_hip_compress does: staged to HBM and validated there), and the
+ * blob is written to the output file.  Errors and messages are the reference's, in
  * its order: "cannot open permutation input file" (BENES_HIP_RUNTIME),
  * "permutation input has an invalid element width or size"
  * (BENES_HIP_INVALID_ARGUMENT), "failed to read permutation input file"
  * (BENES_HIP_RUNTIME), the compress errors, then "failed to write compressed
  * permutation file" (BENES_HIP_RUNTIME).  The output is created or truncated
- * only after compression succeeded.  *out_size receives the blob's size.
- * This build returns BENES_HIP_RUNTIME ("compress_file is not available in
- * this build") without touching either file: the streaming path is switched
- * off. */
+ * only after compression succeeded.  *out_size receives the blob's size. */
 int benes_hip_compress_file(const char* input_path, const char* output_path,
                             uint32_t width_bytes, const benes_hip_format* format,
                             uint64_t* out_size, double* device_seconds, char* err,

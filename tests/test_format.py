@@ -153,15 +153,6 @@ def test_keyed_generator_is_a_bijection():
     assert (a != np.arange(5000)).sum() > 4900   # no structure left to speak of
 
 
-def test_compress_file_reports_it_is_switched_off(tmp_path):
-    # The file path answers with an error and leaves both files alone.
-    src, dst = tmp_path / "perm.bin", tmp_path / "blob.bin"
-    np.arange(8, dtype=np.uint32).tofile(src)
-    with pytest.raises(RuntimeError, match="compress_file is not available"):
-        B.compress_file(src, dst, 4)
-    assert not dst.exists() and src.stat().st_size == 32
-
-
 def test_uniform_parallel_generator():
     # A permutation for every size, the same for both widths, and (with several
     # buckets, n >= 2^19) no position favoured for a given value.

@@ -1,9 +1,12 @@
 # SPDX-License-Identifier: Apache-2.0
 """File ingest and egress (the reference's compress_file, src/benes.cpp:
-498-553): raw little-endian uint32/uint64 files stream through pinned memory
-into HBM, and blobs stream from HBM into the output file.  The blobs equal the
-oracle's and the reference digests; the reference's errors come back in its
-order with its messages, and a failed compress leaves no output file."""
+498-553): raw little-endian uint32/uint64 files are read whole, compressed
+through the host path of benes_hip_compress, and the blob written out.  The
+blobs equal the oracle's and the reference digests; the reference's errors
+come back in its order with its messages, and a failed compress leaves no
+output file.  (An earlier build streamed both files through the pinned ring;
+one run of it stopped making progress, the cause was not found, and that code
+is gone.)"""
 from __future__ import annotations
 
 import hashlib
@@ -16,12 +19,7 @@ import pytest
 import support as S
 
 B = S.benes_hip
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="compress_file's streaming path is switched off: a run of "
-                            "test_file_matches_oracle[47-8-5-2] stopped making progress and "
-                            "its cause has not been found"),
-]
+pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module", autouse=True)
