@@ -553,7 +553,8 @@ def main():
     prof_steps = 3
     for _ in range(prof_steps):
         step()
-    stats = {k: v for k, v in B.kernel_stats().items() if k.startswith("k_")}
+    all_stats = B.kernel_stats()
+    stats = {k: v for k, v in all_stats.items() if k.startswith("k_")}
     B.set_profiling(False)
     total_ms = sum(ms for ms, _ in stats.values())
     dom = max(stats, key=lambda k: stats[k][0])
@@ -600,6 +601,9 @@ def main():
             "launches_per_step": dom_launches / prof_steps,
         },
         "kernel_ms_per_step": {k: v[0] / prof_steps for k, v in sorted(stats.items())},
+        # Union relations overwritten by a concurrent plain-store hook and
+        # joined again by k_union_verify, per compress.
+        "union_rejoined_per_step": all_stats.get("union_rejoined", (0, 0))[1] / prof_steps,
         "pipeline_roofline": {
             "algorithmic_bytes": 4 * N + 8 * N * 11 + size,
             "kernel_ms": total_ms / prof_steps,

@@ -52,6 +52,7 @@ EXPORTS = (
     "benes_hip_set_subtree_shape",
     "benes_hip_set_hook_binning",
     "benes_hip_set_wide_words",
+    "benes_hip_set_plain_union",
     "benes_hip_set_colouring",
     "benes_hip_lookup",
     "benes_hip_lookup_device",
@@ -184,6 +185,7 @@ def library() -> ctypes.CDLL:
     lib.benes_hip_set_subtree_shape.argtypes = [u32, u32]
     lib.benes_hip_set_hook_binning.argtypes = [ctypes.c_int]
     lib.benes_hip_set_wide_words.argtypes = [ctypes.c_int]
+    lib.benes_hip_set_plain_union.argtypes = [ctypes.c_int]
     lib.benes_hip_set_colouring.argtypes = [ctypes.c_int]
     lib.benes_hip_compress_device_noalloc.argtypes = [vp, u64, u32, fp, vp, u64,
                                                       ctypes.POINTER(u64), vp, cp, sz]
@@ -620,6 +622,15 @@ def set_wide_words(mode: int = 0) -> None:
     every level of every call (the blob is the same in both modes)."""
     if library().benes_hip_set_wide_words(mode) != OK:
         raise ValueError(f"no wide-words mode {mode}")
+
+
+def set_plain_union(mode: int = -1) -> None:
+    """Union pass of the depths whose nodes keep to one XCD: -1 by size
+    (default: plain-store hooks, checked and rejoined, where a node's words
+    fit in an XCD's L2), 0 compare-and-swap hooks everywhere, 1 plain-store
+    hooks at every such depth (the blob is the same in every mode)."""
+    if library().benes_hip_set_plain_union(mode) != OK:
+        raise ValueError(f"no plain-union mode {mode}")
 
 
 UNION_FIND, RULERS = 0, 1

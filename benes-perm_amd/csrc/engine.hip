@@ -55,6 +55,21 @@ std::atomic<int> g_hook_binning{-1};
 // 1 at every HBM level of every call (a test setting).
 std::atomic<int> g_wide_words{0};
 
+// Union pass of the HBM levels whose nodes keep to one XCD: -1 by size (the
+// default: plain-store hooks, checked afterwards, where a node holds at most
+// kPlainUnionMaxBytes of words), 0 compare-and-swap hooks everywhere, 1
+// plain-store hooks at every such depth (a test setting).
+std::atomic<int> g_plain_union{-1};
+// Measured on uniform permutations of 2^26 uint32 and 2^32 + 12345 uint64
+// entries: plain-store hooks pay for their check from nodes of 8 MB of words
+// down (2^26: depths 4-9, 27.8 -> 25.7 ms per compress; 16 MB nodes break
+// even), with each XCD uniting nodes of about 2 MB of words at a time (whole
+// L2-resident nodes of 256 KB with all of an XCD's blocks on one node lose a
+// tenth of their hooks to races; interleaving more nodes lets their words
+// leave L2).
+constexpr uint64_t kPlainUnionMaxBytes = uint64_t{8} << 20;
+constexpr uint64_t kPlainUnionGroupBytes = uint64_t{2} << 20;
+
 // Colouring of the HBM levels: 0 the union-find (the default), 1 the ruler
 // orbit minima (an independent second colouring; the blob is the same).
 std::atomic<int> g_colouring{0};
@@ -317,6 +332,16 @@ struct Profiler {
     bool on = false;
     hipStream_t stream = nullptr;
     std::vector<Pending> pending;
+    // Device counters read after the call, added to the table as launches.
+    struct Counter {
+        const char* name;
+        const unsigned long long* dev;
+        size_t words;
+    };
+    std::vector<Counter> counters;
+    void count(const char* name, const unsigned long long* dev, size_t words) {
+        if (on) counters.push_back(Counter{name, dev, words});
+    }
     void begin(const char* name) {
         if (!on) return;
         Pending p{name, nullptr, nullptr};
@@ -348,6 +373,19 @@ struct Profiler {
             (void)hipEventDestroy(p.end);
         }
         pending.clear();
+        for (const Counter& c : counters) {
+            std::vector<unsigned long long> v(c.words);
+            BH_CHECK(hipMemcpy(v.data(), c.dev, sizeof(unsigned long long) * c.words,
+                               hipMemcpyDeviceToHost));
+            auto it = std::find_if(g_prof_stats.begin(), g_prof_stats.end(),
+                                   [&](const KernelStat& k) { return k.name == c.name; });
+            if (it == g_prof_stats.end()) {
+                g_prof_stats.push_back(KernelStat{c.name, 0, 0});
+                it = g_prof_stats.end() - 1;
+            }
+            for (unsigned long long x : v) it->launches += x;
+        }
+        counters.clear();
     }
 };
 
@@ -442,6 +480,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
     const int hook_binning = g_hook_binning.load();
     const bool force_wide = g_wide_words.load() == 1;
     const bool rulers = g_colouring.load() == 1;
+    const int plain_union = g_plain_union.load();
     const uint32_t count = static_cast<uint32_t>(jobs.size());
     uint64_t total_n = 0, total_words = 0;
     uint32_t max_df = 0;
@@ -489,6 +528,8 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         Launch hbm, lds, pack, narrow;  // pack, narrow: both sides' segments
         Launch bin;                     // k_hook_bin: the HBM nodes, 4096 pairs per block
         bool wide_uf = false;
+        bool plain = false;             // union hooks by plain stores, then checked
+        uint32_t plain_group = 1;       // nodes each XCD unites at a time
         bool binned = false;            // first hooks staged in bins of 2^bin_log2 words
         uint32_t bin_log2 = kHookBinMinLog2;
         uint64_t bins = 0, scatter_blocks = 0;
@@ -563,6 +604,14 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                      : hook_binning < 0 ? node_bytes >= (uint64_t{64} << 20) &&
                                               total_n >= (uint64_t{1} << 31)
                                         : false);
+        // Plain-store hooks where every node keeps to one XCD (8 nodes or
+        // more per item) and, by default, its words fit in the XCD's L2.
+        lp.plain = lp.hbm.count && d >= 3 && !lp.wide_uf && !lp.binned && !rulers &&
+                   (plain_union == 1 ||
+                    (plain_union < 0 && node_bytes <= kPlainUnionMaxBytes));
+        while (lp.plain_group < 64 && node_bytes * lp.plain_group * 2 <= kPlainUnionGroupBytes) {
+            lp.plain_group *= 2;
+        }
         if (lp.binned) {
             lp.bin = bin.commit(table);
             // The smallest bins (from 1 MB) a node spans at most kHookBinSpan of.
@@ -650,7 +699,9 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         ws.srows.ensure(8 * max_rows);
         ws.trows.ensure(8 * max_rows);
         ws.deferred.ensure(sizeof(DeferredEdge) * defer_cap);
-        ws.deferred_counts.ensure(sizeof(unsigned long long) * 4 * max_df);
+        // Per depth: 4 counters, then (after all depths) 64 counters of the
+        // relations k_union_verify joined again (counted when profiling).
+        ws.deferred_counts.ensure(sizeof(unsigned long long) * (4 + 64) * max_df);
         if (fill_words) ws.bin_fill.ensure(sizeof(uint32_t) * fill_words);
         if (rulers) ws.rulers.ensure((sizeof(uint64_t) + sizeof(uint32_t)) * 2 * ruler_buckets);
     }
@@ -670,7 +721,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
     fill(blob_ptr, total_words * 2, 0);
     fill(ws.inv0.ptr, total_n * (wide_inverse ? 2 : 1), 0xFFFFFFFFu);
     if (max_df > 0) {
-        fill(ws.deferred_counts.ptr, 2 * 4 * max_df, 0);
+        fill(ws.deferred_counts.ptr, 2 * (4 + 64) * max_df, 0);
         if (fill_words) fill(ws.bin_fill.ptr, fill_words, 0);
     }
 
@@ -827,9 +878,30 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                     }
                 }
                 DeferredEdge* const deferred = ws.deferred.as<DeferredEdge>();
-                BH_LAUNCH(prof, w64 ? "k_union_w64" : "k_union", (k_union<VI, W>), grid(lp.hbm),
-                          dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc, deferred,
-                          dcount, defer_cap, errs);
+                bool plain_done = false;
+                if constexpr (std::is_same_v<W, uint32_t>) {
+                    if (lp.plain) {
+                        // Hook records in this depth's inverse output, which
+                        // only k_route_outputs and k_level_lds write, later.
+                        uint64_t* const rec = reinterpret_cast<uint64_t*>(idst);
+                        unsigned long long* const rejoined =
+                            ws.deferred_counts.as<unsigned long long>() + 4 * max_df + 64 * d;
+                        BH_LAUNCH(prof, "k_union_plain", (k_union_plain<VI>), grid(lp.hbm),
+                                  dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
+                                  rec, deferred, dcount, defer_cap, lp.plain_group, errs);
+                        BH_LAUNCH(prof, "k_union_verify", (k_union_verify<VI>), grid(lp.hbm),
+                                  dim3(kVerifyThreads), 0, st, segp(lp.hbm), lp.hbm.count, uf,
+                                  desc, rec, deferred, dcount, defer_cap, lp.plain_group,
+                                  prof.on ? rejoined : nullptr, errs);
+                        prof.count("union_rejoined", rejoined, 64);
+                        plain_done = true;
+                    }
+                }
+                if (!plain_done) {
+                    BH_LAUNCH(prof, w64 ? "k_union_w64" : "k_union", (k_union<VI, W>),
+                              grid(lp.hbm), dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv,
+                              uf, desc, deferred, dcount, defer_cap, errs);
+                }
                 // Returns at once unless k_union deferred an edge.
                 const dim3 flat_grid(static_cast<uint32_t>(std::min<uint64_t>(lp.hbm.blocks, 2048)));
                 if (spread) {
@@ -1933,6 +2005,12 @@ int benes_hip_set_subtree_shape(uint32_t log2_entries, uint32_t threads) {
 int benes_hip_set_hook_binning(int mode) {
     if (mode < -1 || mode > 1) return BENES_HIP_INVALID_ARGUMENT;
     g_hook_binning.store(mode);
+    return BENES_HIP_OK;
+}
+
+int benes_hip_set_plain_union(int mode) {
+    if (mode < -1 || mode > 1) return BENES_HIP_INVALID_ARGUMENT;
+    g_plain_union.store(mode);
     return BENES_HIP_OK;
 }
 

@@ -281,6 +281,22 @@ __device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uin
     return block_to_node_at(sg, blockIdx.x, node, chunk, spread);
 }
 
+// Segments of 8 nodes or more: each XCD visits its nodes `group` at a time,
+// their blocks interleaved.
+__device__ __forceinline__ bool block_to_node_grouped(const Seg& sg, uint64_t& node,
+                                                      uint64_t& chunk, uint32_t group) {
+    const uint32_t b = static_cast<uint32_t>(blockIdx.x - sg.block0);
+    const uint32_t nodes = static_cast<uint32_t>(sg.nodes);
+    const uint32_t chunks = static_cast<uint32_t>(sg.chunks);
+    const uint32_t xcd = b % kXcds, local = b / kXcds, per_xcd = nodes / kXcds;
+    const uint32_t g = per_xcd < group ? per_xcd : group;
+    const uint32_t grp = local / (g * chunks), in_grp = local - grp * (g * chunks);
+    const uint32_t n = (grp * g + in_grp % g) * kXcds + xcd, c = in_grp / g;
+    node = n;
+    chunk = c;
+    return n < nodes && c < chunks;
+}
+
 // Node descriptors of every depth above the fusion depth, in one launch:
 // each segment is one item at one depth (`depth`), its descriptors at desc0.
 __global__ void k_describe(const ItemCtx* __restrict__ items, const Seg* __restrict__ segs,
@@ -777,6 +793,134 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
     }
     uf_union_rem<DeviceForest>(u, w >> 1, static_cast<uint32_t>(w & 1), lo, 0u, e.label,
                                ~uint64_t{0});
+}
+
+// The union pass without compare-and-swaps, for depths whose nodes each keep
+// to one XCD and whose words fit its L2.  The blocks of an XCD take its nodes
+// one after another, so the walks' loads and the hooks' stores stay in L2,
+// where a compare-and-swap would go past it to memory (global atomics on this
+// device all execute there, at about a third of the rate of random loads).
+// Hooks are plain stores: two walks that find the same root at once may both
+// store to it, and only the later relation stays.  Each thread therefore
+// records the hook it made, (hooked slot << 32) | word, at
+// rec[(off + 2q) >> 1] (distinct for every pair of every node), or
+// kNoHook; k_union_verify then joins again the relations that were
+// overwritten.  Every stored word is a true relation to a smaller slot, so
+// the walks of this pass stay correct whatever mix of old and new words they
+// read; a walk that relied on an overwritten relation is covered once that
+// relation is joined again, since the relations it followed are facts.
+constexpr uint64_t kNoHook = ~uint64_t{0};
+
+template <typename VI>
+__global__ __launch_bounds__(kBlock) void k_union_plain(const Seg* __restrict__ segs,
+                                                        uint32_t nsegs,
+                                                        const VI* __restrict__ inv,
+                                                        uint32_t* __restrict__ uf,
+                                                        const NodeDesc* __restrict__ desc,
+                                                        uint64_t* __restrict__ rec,
+                                                        DeferredEdge* __restrict__ deferred,
+                                                        unsigned long long* __restrict__ deferred_count,
+                                                        uint64_t defer_cap, uint32_t group,
+                                                        const int* __restrict__ errs) {
+    using W = uint32_t;
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    uint64_t node, chunk;
+    if (!block_to_node_grouped(sg, node, chunk, group)) return;
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, cnt = nd.cnt;
+    const uint64_t q = chunk * kBlock + threadIdx.x;
+    if (q >= (cnt >> 1)) return;
+    const PairEdge e = pair_edge(inv, off, cnt, q);
+    uint64_t r = kNoHook;
+    if (e.lo != e.hi && e.hi_odd) {
+        W* u = uf + off;
+        const W lo = static_cast<W>(e.lo), hi = static_cast<W>(e.hi);
+        const W rel = (lo << 1) | e.label;
+        const W w = DeviceForest::load(u + hi);
+        if (w == UfWord<W>::root) {
+            DeviceForest::store(u + hi, rel);
+            r = (static_cast<uint64_t>(hi) << 32) | rel;
+        } else if (w != rel) {
+            W hooked = 0, word = 0;
+            int got = uf_union_rem_plain<DeviceForest>(u, w >> 1, static_cast<uint32_t>(w & 1), lo,
+                                                       0u, e.label, kWalkCap, hooked, word);
+            if (got < 0) {
+                // A long path: joined after k_flatten, unless the list is
+                // full; then walked to the end here (each step lowers the
+                // larger of the two vertices, so fewer than 2^32 steps).
+                const unsigned long long i = atomicAdd(deferred_count, 1ull);
+                if (i < defer_cap) {
+                    deferred[i] = DeferredEdge{off, e.lo | (static_cast<uint64_t>(e.label) << 63), e.hi};
+                } else {
+                    got = uf_union_rem_plain<DeviceForest>(u, w >> 1, static_cast<uint32_t>(w & 1),
+                                                           lo, 0u, e.label, 0xFFFFFFFFu, hooked,
+                                                           word);
+                }
+            }
+            if (got > 0) r = (static_cast<uint64_t>(hooked) << 32) | word;
+        }
+    }
+    __builtin_nontemporal_store(r, rec + ((off + 2 * q) >> 1));
+}
+
+// After k_union_plain: each recorded hook is checked against its word, and a
+// relation that a later hook overwrote is joined again, with
+// compare-and-swap hooks (the slot has some smaller parent by now, so the
+// relation is recorded between the two trees).  The blocks take the nodes in
+// the order k_union_plain did, so each node's words are read while in L2.
+// One wave per block of kBlock pairs: each lane issues the loads of its four
+// records, then of the four words they name, before comparing any, so that
+// the wave waits for memory twice instead of eight times.
+constexpr uint32_t kVerifyThreads = 64;
+constexpr uint32_t kVerifyPairs = kBlock / kVerifyThreads;
+
+template <typename VI>
+__global__ __launch_bounds__(kVerifyThreads) void k_union_verify(
+    const Seg* __restrict__ segs, uint32_t nsegs, uint32_t* __restrict__ uf,
+    const NodeDesc* __restrict__ desc, const uint64_t* __restrict__ rec,
+    DeferredEdge* __restrict__ deferred, unsigned long long* __restrict__ deferred_count,
+    uint64_t defer_cap, uint32_t group, unsigned long long* __restrict__ rejoined,
+    const int* __restrict__ errs) {
+    using W = uint32_t;
+    const Seg& sg = find_seg(segs, nsegs);
+    if (errs[sg.item]) return;
+    uint64_t node, chunk;
+    if (!block_to_node_grouped(sg, node, chunk, group)) return;
+    const NodeDesc& nd = desc[sg.desc0 + node];
+    const uint64_t off = nd.off, h = nd.cnt >> 1;
+    W* const u = uf + off;
+    uint64_t r[kVerifyPairs];
+#pragma unroll
+    for (uint32_t k = 0; k < kVerifyPairs; ++k) {
+        const uint64_t q = chunk * kBlock + k * kVerifyThreads + threadIdx.x;
+        r[k] = q < h ? __builtin_nontemporal_load(rec + ((off + 2 * q) >> 1)) : kNoHook;
+    }
+    W w[kVerifyPairs];
+#pragma unroll
+    for (uint32_t k = 0; k < kVerifyPairs; ++k) {
+        w[k] = r[k] != kNoHook ? DeviceForest::load(u + static_cast<W>(r[k] >> 32)) : 0u;
+    }
+    uint32_t lost = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kVerifyPairs; ++k) {
+        if (r[k] == kNoHook || w[k] == static_cast<W>(r[k])) continue;
+        ++lost;
+        const W x = static_cast<W>(r[k] >> 32), word = static_cast<W>(r[k]);
+        // value(x) ^ value(y) == p for the relation x -> (y, p).
+        if (uf_union_rem<DeviceForest>(u, x, 0u, word >> 1, 0u, word & 1u, kWalkCap)) continue;
+        const unsigned long long i = atomicAdd(deferred_count, 1ull);
+        if (i < defer_cap) {
+            deferred[i] = DeferredEdge{off, (word >> 1) | (static_cast<uint64_t>(word & 1u) << 63), x};
+        } else {
+            uf_union_rem<DeviceForest>(u, x, 0u, word >> 1, 0u, word & 1u, ~uint64_t{0});
+        }
+    }
+    if (rejoined) {
+        // Counted when profiling, into one of 64 counters per depth.
+        const unsigned long long total = wave_inclusive_sum(lost);
+        if (lane_id() == 63 && total) atomicAdd(rejoined + (blockIdx.x & 63), total);
+    }
 }
 
 // Runs only when k_union deferred an edge at this depth (`counts[0]`):

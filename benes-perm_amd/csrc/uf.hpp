@@ -288,6 +288,43 @@ __device__ __forceinline__ bool uf_union_rem(W* uf, W x, uint32_t px, W y, uint3
     return true;
 }
 
+// uf_union_rem whose hook is a plain store instead of a compare-and-swap,
+// for a union pass that checks its hooks afterwards.  Two walks that find
+// the same root at once may both store to it, and the later store wins; the
+// earlier relation is then missing from the forest.  Every stored word is
+// still a true relation to a smaller slot, so the forest stays valid, and
+// the caller keeps the hook it made (`hooked` slot, `word` stored) so that a
+// later pass can find the ones that were overwritten and join them again.
+// Returns 1 after a hook, 0 when the two vertices met (already joined) and
+// -1 when it stopped after `max_steps` parent words without storing.
+template <typename F, typename W>
+__device__ __forceinline__ int uf_union_rem_plain(W* uf, W x, uint32_t px, W y, uint32_t py,
+                                                  uint32_t label, uint32_t max_steps, W& hooked,
+                                                  W& word) {
+    constexpr W kRoot = UfWord<W>::root;
+    for (uint32_t step = 0; x != y; ++step) {
+        if (step == max_steps) return -1;
+        if (x < y) {
+            const W t = x;
+            x = y;
+            y = t;
+            const uint32_t tp = px;
+            px = py;
+            py = tp;
+        }
+        const W w = F::load(uf + x);
+        if (w == kRoot) {
+            word = (y << 1) | static_cast<W>((px ^ py ^ label) & 1);
+            F::store(uf + x, word);
+            hooked = x;
+            return 1;
+        }
+        px ^= static_cast<uint32_t>(w & 1);
+        x = w >> 1;
+    }
+    return 0;
+}
+
 // Slot numbering inside a node of `count` entries (h = count/2 pairs).
 template <typename W>
 __device__ __forceinline__ W input_slot(uint64_t x, uint64_t count) {

@@ -328,6 +328,14 @@ int benes_hip_set_hook_binning(int mode);
  * (src/metal/benes_metal.mm:615-633). */
 int benes_hip_set_wide_words(int mode);
 
+/* Union pass of the HBM levels whose nodes each keep to one XCD: -1 by size
+ * (the default: hooks by plain stores, checked afterwards and the overwritten
+ * relations joined again, where a node's union-find words fit in an XCD's
+ * L2), 0 compare-and-swap hooks everywhere, 1 plain-store hooks at every such
+ * depth.  The blob is the same in every mode; tests use 1 and 0 to check both
+ * paths at small sizes.  No counterpart in the reference. */
+int benes_hip_set_plain_union(int mode);
+
 /* Colouring of the levels coloured in HBM (nodes of more than 2^16 entries):
  * 0 the parity union-find (the default), 1 ruler orbit minima, a second
  * colouring that shares no code with the union-find: each entry's colour is

@@ -1,0 +1,119 @@
+# SPDX-License-Identifier: Apache-2.0
+"""The union pass of the depths whose nodes keep to one XCD, in both of its
+forms: compare-and-swap hooks (k_union) and plain-store hooks checked and
+rejoined afterwards (k_union_plain + k_union_verify), each forced at every
+such depth with benes_hip_set_plain_union.  By default the plain form runs
+only where a node's words fit in an XCD's L2.  Both must give the canonical
+colouring of the reference (src/benes.cpp:247-297), so every blob is checked
+byte for byte against the oracle or a digest of the reference library's own
+blob, on random, chain-structured and batched inputs, and a profile shows
+which union kernels ran."""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import support as S
+
+B = S.benes_hip
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=[0, 1], ids=["cas", "plain"])
+def mode(request):
+    assert B.available(), "no gfx950 device or libbenes_hip.so failed to load its kernels"
+    B.set_plain_union(request.param)
+    yield request.param
+    B.set_plain_union(-1)
+
+
+def compress_profiled(perm, fmt):
+    B.set_profiling(True)
+    try:
+        blob = B.compress(perm, fmt)
+        stats = B.kernel_stats()
+    finally:
+        B.set_profiling(False)
+    return blob, stats
+
+
+def local_depths(n: int) -> int:
+    """HBM depths (largest node above 2^16 entries) from depth 3 on, where
+    every node keeps to one XCD."""
+    d, count = 0, 0
+    while -(-n // (1 << d)) > (1 << 16):
+        count += d >= 3
+        d += 1
+    return count
+
+
+def assert_union_kernels(stats, mode, local_depths):
+    """With `local_depths` HBM depths whose nodes keep to one XCD, mode 1 runs
+    k_union_plain + k_union_verify there and mode 0 never does."""
+    plain = stats.get("k_union_plain", (0, 0))[1]
+    verify = stats.get("k_union_verify", (0, 0))[1]
+    if mode == 1:
+        assert plain == local_depths and verify == local_depths, stats
+        print(f"  relations joined again: {stats.get('union_rejoined', (0, 0))[1]}", flush=True)
+    else:
+        assert plain == 0 and verify == 0, stats
+
+
+@pytest.mark.parametrize("n,g,pl,ic,oc,width", [
+    (1 << 20, 5, 0, (), (), 4),                 # depth 3 only (2^17-entry nodes)
+    ((1 << 21) + 12345, 5, 2, (), (), 8),
+    ((1 << 22) + 1, 4, 1, (10, 9), (6, 13), 4),
+    (1 << 23, 5, 2, (), (), 8),
+])
+def test_random_matches_oracle(mode, n, g, pl, ic, oc, width):
+    perm = B.generate(B.FISHER_YATES, n, n ^ 0x9A1, 0, width)
+    print(f"union mode={mode} random n={n} g={g} pl={pl} width={width}", flush=True)
+    got, stats = compress_profiled(perm, S.fmt(g, pl, ic, oc))
+    assert_union_kernels(stats, mode, local_depths(n))
+    assert got == S.oracle_compress(perm, g, pl, ic, oc)
+
+
+@pytest.mark.parametrize("kind", ["reverse", "rotate1", "rotate_half", "shuffle", "reverse_halves"])
+def test_structured_matches_oracle(mode, kind):
+    # Chains of neighbouring slots: walks deferred past 1024 steps, then the
+    # flatten and the deferred joins, after the plain hooks were checked.
+    n = (1 << 22) + 1
+    perm = S.structured(kind, n)
+    print(f"union mode={mode} structured {kind} n={n}", flush=True)
+    got, stats = compress_profiled(perm.astype(np.uint32), S.fmt(5, 1))
+    assert_union_kernels(stats, mode, local_depths(n))
+    assert got == S.oracle_compress(perm, 5, 1)
+
+
+def test_mixed_batch(mode):
+    # Items of different depths and offsets (odd ones among them) share the
+    # launches; each item's records stay within its own nodes.
+    rng = np.random.default_rng(0x9A1)
+    perms = [rng.permutation(n).astype(np.uint64 if w == 8 else np.uint32)
+             for n, w in (((1 << 21) + 3, 4), ((1 << 20) + 7, 8), (70001, 4), (33, 8),
+                          ((1 << 19) + 1, 4))]
+    perms.append(S.structured("rotate1", (1 << 20) + 1).astype(np.uint32))
+    fmts = [S.fmt(5, 0), S.fmt(4, 2), S.fmt(3, 1), S.fmt(5, 2), S.fmt(5, 1), S.fmt(5, 0)]
+    print(f"union mode={mode} mixed batch", flush=True)
+    out = B.compress_batch(perms, fmts)
+    for p, f, blob in zip(perms, fmts, out):
+        assert blob == S.oracle_compress(p.astype(np.uint64), f.middle_group_log2,
+                                         f.middle_placement)
+
+
+def fixtures():
+    with open(os.path.join(S.GOLDEN, "fixtures.json")) as f:
+        return [r for r in json.load(f) if r["n"] > (1 << 16)]
+
+
+@pytest.mark.parametrize("rec", fixtures(), ids=lambda r: r["name"])
+def test_reference_digests(mode, rec):
+    perm = B.generate(rec["kind"], rec["n"], rec["seed"], rec["rate"], rec["width"])
+    print(f"union mode={mode} digest {rec['name']}", flush=True)
+    blob = B.compress(perm, S.fmt(rec["g"], rec["placement"], rec["ic"], rec["oc"]))
+    assert len(blob) == rec["size"]
+    assert hashlib.sha256(blob).hexdigest() == rec["sha256"], rec["name"]
