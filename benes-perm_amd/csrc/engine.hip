@@ -68,6 +68,7 @@ std::atomic<int> g_plain_union{-1};
 // tenth of their hooks to races; interleaving more nodes lets their words
 // leave L2).
 constexpr uint64_t kPlainUnionMaxBytes = uint64_t{8} << 20;
+
 constexpr uint64_t kPlainUnionGroupBytes = uint64_t{2} << 20;
 
 // Colouring of the HBM levels: 0 the union-find (the default), 1 the ruler
@@ -529,7 +530,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         Launch bin;                     // k_hook_bin: the HBM nodes, 4096 pairs per block
         bool wide_uf = false;
         bool plain = false;             // union hooks by plain stores, then checked
-        uint32_t plain_group = 1;       // nodes each XCD unites at a time
+        uint32_t plain_group = 1;       // nodes (~2 MB of words) each XCD unites or routes at a time
         bool binned = false;            // first hooks staged in bins of 2^bin_log2 words
         uint32_t bin_log2 = kHookBinMinLog2;
         uint64_t bins = 0, scatter_blocks = 0;
@@ -816,7 +817,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                           errs);
                 BH_LAUNCH(prof, sizeof(VI) == 8 ? "k_route_outputs_w64" : "k_route_outputs",
                           (k_route_outputs<VI>), grid(lp.hbm), dim3(kBlock), 0, st, segp(lp.hbm),
-                          lp.hbm.count, inv, idst, desc, srows, trows, errs);
+                          lp.hbm.count, inv, idst, desc, srows, trows, nullptr, 0u, errs);
             } else if (lp.hbm.count) {
                 // Per depth: edges deferred, flatten blocks finished, hooks
                 // under the slot just below.
@@ -850,6 +851,10 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                                   st, uf, pdst, idst, fill, lp.bins, lp.bin_log2);
                     }
                 }
+                // The first hooks store only the words that get a parent:
+                // the others start as root words, set at the root here and
+                // below it by the previous depth's k_route_outputs.
+                if (d == 0 && !lp.binned) fill(uf, total_n * sizeof(W) / 4, 0xFFFFFFFFu);
                 if (lp.binned) {
                     // Hooked above.
                 } else if (spread) {
@@ -918,11 +923,16 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                 BH_LAUNCH(prof, w64 ? "k_route_inputs_w64" : "k_route_inputs", (k_route_inputs<W>),
                           grid(lp.hbm), dim3(kBlock),
                           0, st, ditems, segp(lp.hbm), lp.hbm.count, level0, plev, pdst, uf, desc,
-                          srows, dcount + 2, chain_threshold, errs);
+                          srows, dcount + 2, chain_threshold, spread ? 0u : lp.plain_group,
+                          errs);
+                // Root words for the next depth's first hooks.
+                const bool next_hooks = d + 1 < max_df && levels[d + 1].hbm.count &&
+                                        !levels[d + 1].binned;
                 BH_LAUNCH(prof, sizeof(VI) == 8 ? "k_route_outputs_w64" : "k_route_outputs",
                           (k_route_outputs<VI>), grid(lp.hbm),
                           dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, idst, desc, srows,
-                          trows, errs);
+                          trows, next_hooks ? ws.uf.ptr : nullptr,
+                          next_hooks && levels[d + 1].wide_uf ? 1u : 0u, errs);
             }
             if (lp.lds.count) {
                 BH_LAUNCH(prof, sizeof(VI) == 8 ? "k_level_lds_w64" : "k_level_lds",

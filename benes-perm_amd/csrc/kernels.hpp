@@ -297,6 +297,14 @@ __device__ __forceinline__ bool block_to_node_grouped(const Seg& sg, uint64_t& n
     return n < nodes && c < chunks;
 }
 
+// block_to_node_grouped where `group` is nonzero (segments of 8 nodes or
+// more), else the node-by-node mapping of block_to_node.
+__device__ __forceinline__ bool block_to_node_g(const Seg& sg, uint64_t& node, uint64_t& chunk,
+                                                uint32_t group) {
+    return group ? block_to_node_grouped(sg, node, chunk, group)
+                 : block_to_node(sg, node, chunk, false);
+}
+
 // Node descriptors of every depth above the fusion depth, in one launch:
 // each segment is one item at one depth (`depth`), its descriptors at desc0.
 __global__ void k_describe(const ItemCtx* __restrict__ items, const Seg* __restrict__ segs,
@@ -328,10 +336,11 @@ __device__ __forceinline__ uint64_t in_slot(uint64_t x, uint64_t count) {
 // Every input slot has exactly one even entry (2p for pair p, the even tail
 // entry count - 1 for slot 0 of an odd node), and each entry reaches one
 // output pair, whose edge joins the entry's slot to the other entry's slot.
-//   k_hook_first   each slot's word is written by the edge of its even entry:
-//                  the relation to the edge's other slot when that slot is
-//                  smaller, else the root word.  One plain store per slot, no
-//                  atomics; it replaces a fill of root words.
+//   k_hook_first   the words start as root words (set by the previous
+//                  depth's k_route_outputs, or a fill at the root); each slot
+//                  whose even entry's edge leads to a smaller slot stores
+//                  that relation.  One plain store for about half the slots,
+//                  no atomics.
 //   k_union        the edges reached by their larger slot's odd entry (the
 //                  others were settled by the first pass, which hooked the
 //                  larger slot through that very edge).  A larger slot the
@@ -367,7 +376,7 @@ __device__ __forceinline__ PairEdge pair_edge(const VI* inv, uint64_t off, uint6
 
 // Only slots s with (s >> part_shift) == part are written (a node whose
 // words would just overflow an XCD's L2 is hooked in two passes, each
-// writing half of them), so every slot is written by exactly one pass.
+// writing half of them), so every slot is written by at most one pass.
 // With kCount (depths whose nodes spread over several XCDs), every eighth
 // block counts its slots hooked under the slot just below them (`adjacent`),
 // up to just past `chain_threshold` (a wave adds its count only while the
@@ -398,21 +407,15 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
         const uint64_t b = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q + 1));
         const uint64_t sa = in_slot(a, cnt), sb = in_slot(b, cnt);
         const W label = static_cast<W>(1 ^ ((a ^ b) & 1));
-        if (!(a & 1) && (sa >> part_shift) == part) {
-            u[sa] = sb < sa ? (static_cast<W>(sb) << 1) | label : UfWord<W>::root;
+        if (!(a & 1) && sb < sa && (sa >> part_shift) == part) {
+            u[sa] = (static_cast<W>(sb) << 1) | label;
         }
-        if (!(b & 1) && (sb >> part_shift) == part) {
-            u[sb] = sa < sb ? (static_cast<W>(sa) << 1) | label : UfWord<W>::root;
+        if (!(b & 1) && sa < sb && (sb >> part_shift) == part) {
+            u[sb] = (static_cast<W>(sa) << 1) | label;
         }
         if (kCount && (blockIdx.x & 7) == 0) {
             next_to = (!(a & 1) && sb + 1 == sa) || (!(b & 1) && sa + 1 == sb);
         }
-    } else if ((cnt & 1) && q == h) {
-        // The unpaired output entry: no edge; an even entry there leaves its
-        // slot a root.
-        const uint64_t x = static_cast<uint64_t>(inv[off + cnt - 1]);
-        const uint64_t sx = in_slot(x, cnt);
-        if (!(x & 1) && (sx >> part_shift) == part) u[sx] = UfWord<W>::root;
     }
     if (!kCount || (blockIdx.x & 7) != 0) return;
     const uint64_t ballot = __ballot(next_to);
@@ -985,6 +988,9 @@ __device__ __forceinline__ uint64_t load_perm(const ItemCtx& it, const uint32_t*
 
 // One thread per input pair: canonical switch value s_p (its slot's parity to
 // the root), child routing of P, and s ballot-packed into the node's row.
+// With a nonzero `group` (depths whose nodes keep to one XCD) each XCD takes
+// its nodes `group` at a time, as the plain-store union pass does: the finds
+// of small nodes then spread over several nodes' words, still L2-resident.
 // When the first-hook pass counted more than `chain_threshold` hooks under
 // the slot just below (chains of neighbouring slots) at a depth whose nodes
 // spread over several XCDs, the walks read and write coherently across XCDs,
@@ -1000,11 +1006,12 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restri
                                                          uint64_t* __restrict__ srows,
                                                          const unsigned long long* __restrict__ adjacent,
                                                          unsigned long long chain_threshold,
+                                                         uint32_t group,
                                                          const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
     uint64_t node, chunk;
-    if (!block_to_node(sg, node, chunk, false)) return;
+    if (!block_to_node_g(sg, node, chunk, group)) return;
     const ItemCtx it = items[sg.item];
     const NodeDesc& nd = desc[sg.desc0 + node];
     const uint64_t off = nd.off, cnt = nd.cnt, row = nd.row;
@@ -1040,6 +1047,12 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restri
 
 // One thread per output pair: t_q = colour(I[2q]) read from the input rows,
 // child routing of I, and t ballot-packed into the node's row.
+// With `next_uf` (the next depth colours these nodes' children in HBM), the
+// children's union-find words are set to the root word here, where the
+// parent's words are no longer read: thread q covers slot q of both
+// children (a child of c entries has ceil(c/2) slots, at most the parent's
+// pair count), as 64-bit words where the next depth uses them.  The next
+// depth's first hooks then store only the words that have a parent.
 template <typename VI>
 __global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict__ segs,
                                                           uint32_t nsegs,
@@ -1048,6 +1061,8 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict_
                                                           const NodeDesc* __restrict__ desc,
                                                           const uint64_t* __restrict__ srows,
                                                           uint64_t* __restrict__ trows,
+                                                          void* __restrict__ next_uf,
+                                                          uint32_t next_wide,
                                                           const int* __restrict__ errs) {
     const Seg& sg = find_seg(segs, nsegs);
     if (errs[sg.item]) return;
@@ -1065,6 +1080,18 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict_
         if (!((cnt & 1) && a0 == cnt - 1)) t = row_bit(srows, row, a0 >> 1) ^ (a0 & 1);
         __builtin_nontemporal_store(static_cast<uint32_t>((t ? a1 : a0) >> 1), idst + off + q);
         __builtin_nontemporal_store(static_cast<uint32_t>((t ? a0 : a1) >> 1), idst + off + lc + q);
+        if (next_uf) {
+            const uint64_t lslots = (lc + 1) >> 1, rslots = (h + 1) >> 1;
+            if (next_wide) {
+                uint64_t* const w = static_cast<uint64_t*>(next_uf);
+                if (q < lslots) w[off + q] = UfWord<uint64_t>::root;
+                if (q < rslots) w[off + lc + q] = UfWord<uint64_t>::root;
+            } else {
+                uint32_t* const w = static_cast<uint32_t*>(next_uf);
+                if (q < lslots) w[off + q] = UfWord<uint32_t>::root;
+                if (q < rslots) w[off + lc + q] = UfWord<uint32_t>::root;
+            }
+        }
     } else if ((cnt & 1) && q == h) {
         idst[off + h] = static_cast<uint32_t>(static_cast<uint64_t>(inv[off + cnt - 1]) >> 1);
     }
