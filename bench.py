@@ -64,6 +64,7 @@ def algorithmic_bytes(kernel: str, n: int, blob: int) -> float:
     per_launch = {
         "k_hook_first": 6.0 * n,        # inverse read (4 B/entry), every slot's parent word written (4 B/slot)
         "k_union": 4.0 * n,             # inverse read
+        "k_union_plain": 4.0 * n,       # inverse read (the same pass with plain-store hooks)
         "k_route_inputs": 10.0 * n + n / 8,   # P read, children written, slot words read, row bits
         "k_route_outputs": 8.0 * n + n / 4,   # I read, children written, row bits
         "k_level_lds": 16.0 * n + n / 4,      # P and I read, both children written, row bits
@@ -76,6 +77,16 @@ def algorithmic_bytes(kernel: str, n: int, blob: int) -> float:
         switch_bits_above = global_levels(n) * (n // 2) * 2 / 8.0
         return 4.0 * n + (blob - lay["input_offset"] - switch_bits_above)
     return per_launch.get(kernel, 0.0)
+
+
+def union_pass(stats: dict, prof_steps: int) -> dict:
+    ms = sum(stats.get(k, (0.0, 0))[0] for k in ("k_union", "k_union_plain", "k_union_verify"))
+    depths = (stats.get("k_union", (0, 0))[1] + stats.get("k_union_plain", (0, 0))[1]) / prof_steps
+    per_depth = ms / prof_steps / depths if depths else 0.0
+    achieved = 4.0 * N / (per_depth * 1e-3) / 1e9 if per_depth else 0.0
+    return {"kernels": ["k_union", "k_union_plain", "k_union_verify"],
+            "ms_per_step": ms / prof_steps, "depths": depths, "ms_per_depth": per_depth,
+            "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS}
 
 
 def _median_seconds(fn, runs: int = 5):
@@ -604,6 +615,10 @@ def main():
         # Union relations overwritten by a concurrent plain-store hook and
         # joined again by k_union_verify, per compress.
         "union_rejoined_per_step": all_stats.get("union_rejoined", (0, 0))[1] / prof_steps,
+        # The union pass of every HBM depth together (compare-and-swap
+        # launches, plain-store launches and their checks): per depth, against
+        # its algorithmic bytes (the inverse read, 4 B/entry).
+        "union_pass": union_pass(stats, prof_steps),
         "pipeline_roofline": {
             "algorithmic_bytes": 4 * N + 8 * N * 11 + size,
             "kernel_ms": total_ms / prof_steps,

@@ -700,9 +700,10 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         ws.srows.ensure(8 * max_rows);
         ws.trows.ensure(8 * max_rows);
         ws.deferred.ensure(sizeof(DeferredEdge) * defer_cap);
-        // Per depth: 4 counters, then (after all depths) 64 counters of the
-        // relations k_union_verify joined again (counted when profiling).
-        ws.deferred_counts.ensure(sizeof(unsigned long long) * (4 + 64) * max_df);
+        // Per depth: 4 counters, then (after all depths) kRejoinCounters
+        // counters of the relations k_union_verify joined again (counted
+        // when profiling).
+        ws.deferred_counts.ensure(sizeof(unsigned long long) * (4 + kRejoinCounters) * max_df);
         if (fill_words) ws.bin_fill.ensure(sizeof(uint32_t) * fill_words);
         if (rulers) ws.rulers.ensure((sizeof(uint64_t) + sizeof(uint32_t)) * 2 * ruler_buckets);
     }
@@ -722,7 +723,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
     fill(blob_ptr, total_words * 2, 0);
     fill(ws.inv0.ptr, total_n * (wide_inverse ? 2 : 1), 0xFFFFFFFFu);
     if (max_df > 0) {
-        fill(ws.deferred_counts.ptr, 2 * (4 + 64) * max_df, 0);
+        fill(ws.deferred_counts.ptr, 2 * (4 + kRejoinCounters) * max_df, 0);
         if (fill_words) fill(ws.bin_fill.ptr, fill_words, 0);
     }
 
@@ -890,7 +891,8 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                         // only k_route_outputs and k_level_lds write, later.
                         uint64_t* const rec = reinterpret_cast<uint64_t*>(idst);
                         unsigned long long* const rejoined =
-                            ws.deferred_counts.as<unsigned long long>() + 4 * max_df + 64 * d;
+                            ws.deferred_counts.as<unsigned long long>() + 4 * max_df +
+                            kRejoinCounters * d;
                         BH_LAUNCH(prof, "k_union_plain", (k_union_plain<VI>), grid(lp.hbm),
                                   dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
                                   rec, deferred, dcount, defer_cap, lp.plain_group, errs);
@@ -898,7 +900,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                                   dim3(kVerifyThreads), 0, st, segp(lp.hbm), lp.hbm.count, uf,
                                   desc, rec, deferred, dcount, defer_cap, lp.plain_group,
                                   prof.on ? rejoined : nullptr, errs);
-                        prof.count("union_rejoined", rejoined, 64);
+                        prof.count("union_rejoined", rejoined, kRejoinCounters);
                         plain_done = true;
                     }
                 }

@@ -23,6 +23,12 @@
 //                    one thread per output pair: parity union-find over the
 //                    node's input slots in HBM (plain-store first hooks, then
 //                    one CAS per remaining hook; long walks deferred)
+//   k_union_plain, k_union_verify
+//                    instead of k_union where a node keeps to one XCD and its
+//                    words fit in 8 MB: the hooks are plain stores, each XCD
+//                    uniting about 2 MB of nodes at a time in its L2; the
+//                    check then joins again every relation a concurrent hook
+//                    overwrote
 //   k_hook_bin, k_hook_scatter
 //                    the first hooks of very large nodes instead: staged by
 //                    word bin, then written bin by bin, each bin by one XCD
@@ -31,7 +37,8 @@
 //   k_route_inputs   one thread per input pair: switch values, routing of P,
 //                    values ballot-packed into node-major bit rows
 //   k_route_outputs  one thread per output pair: output switch values from
-//                    the input rows, routing of the inverse
+//                    the input rows, routing of the inverse, root words for
+//                    the children's first hooks
 // Nodes of at most 2^16 entries (LDS levels):
 //   k_level_lds      one workgroup per node: the same three steps in LDS
 // Both kinds of level, per side:
@@ -876,6 +883,7 @@ __global__ __launch_bounds__(kBlock) void k_union_plain(const Seg* __restrict__ 
 // records, then of the four words they name, before comparing any, so that
 // the wave waits for memory twice instead of eight times.
 constexpr uint32_t kVerifyThreads = 64;
+constexpr uint32_t kRejoinCounters = 1024;
 constexpr uint32_t kVerifyPairs = kBlock / kVerifyThreads;
 
 template <typename VI>
@@ -920,9 +928,12 @@ __global__ __launch_bounds__(kVerifyThreads) void k_union_verify(
         }
     }
     if (rejoined) {
-        // Counted when profiling, into one of 64 counters per depth.
+        // Counted when profiling, into one of kRejoinCounters counters per
+        // depth, so that few waves share one.
         const unsigned long long total = wave_inclusive_sum(lost);
-        if (lane_id() == 63 && total) atomicAdd(rejoined + (blockIdx.x & 63), total);
+        if (lane_id() == 63 && total) {
+            atomicAdd(rejoined + (blockIdx.x & (kRejoinCounters - 1)), total);
+        }
     }
 }
 
