@@ -457,6 +457,34 @@ struct SegList {
         segs.push_back(s);
         blocks += blocks_of_item;
     }
+    // One lane group of a level launch: `lanes` nodes, each its own entry
+    // (nodes == 1), sharing 8 * lane_g * max_chunks blocks (level_block).
+    struct Lane {
+        uint32_t item;
+        uint64_t chunks, desc, row0, row_words;
+    };
+    void add_lane_group(const std::vector<Lane>& group, uint32_t lane_g) {
+        blocks = round8(blocks);
+        uint64_t max_chunks = 0;
+        for (const Lane& l : group) max_chunks = std::max(max_chunks, l.chunks);
+        const uint64_t group_blocks = uint64_t{8} * lane_g * max_chunks;
+        for (size_t k = 0; k < group.size(); ++k) {
+            Seg s{};
+            s.block0 = blocks;
+            s.blocks = group_blocks;
+            s.nodes = 1;
+            s.chunks = group[k].chunks;
+            s.desc0 = group[k].desc;
+            s.row0 = group[k].row0;
+            s.row_words = group[k].row_words;
+            s.item = group[k].item;
+            s.lane = static_cast<uint32_t>(k);
+            s.lanes = static_cast<uint32_t>(group.size());
+            s.lane_g = lane_g;
+            segs.push_back(s);
+        }
+        blocks += group_blocks;
+    }
     // Appends the segments to the batch table.
     Launch commit(std::vector<Seg>& table) const {
         Launch l;
@@ -529,6 +557,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         Launch hbm, lds, pack, narrow;  // pack, narrow: both sides' segments
         Launch bin;                     // k_hook_bin: the HBM nodes, 4096 pairs per block
         bool wide_uf = false;
+        bool lanes = false;             // small nodes of many items in lane groups, one XCD each
         bool plain = false;             // union hooks by plain stores, then checked
         uint32_t plain_group = 1;       // nodes (~2 MB of words) each XCD unites or routes at a time
         bool binned = false;            // first hooks staged in bins of 2^bin_log2 words
@@ -546,6 +575,11 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
     for (uint32_t d = 0; d < max_df; ++d) {
         LevelPlan& lp = levels[d];
         SegList hbm, lds, pack, narrow, bin;
+        struct HbmItem {
+            uint32_t item;
+            uint64_t chunks, desc0, row0, row_words;
+        };
+        std::vector<HbmItem> hbm_items;
         uint64_t desc_count = desc_base, row_count = 0;
         const uint64_t nodes = uint64_t{1} << d;
         for (uint32_t i = 0; i < count; ++i) {
@@ -557,7 +591,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                          false, d);
             if (cmax > kLevelLds) {
                 const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
-                hbm.add(i, nodes * chunks, nodes, chunks, desc_count, row_count, row_words, true);
+                hbm_items.push_back(HbmItem{i, chunks, desc_count, row_count, row_words});
                 const uint64_t bin_chunks =
                     ceil_div(maxpairs + 1, uint64_t{kHookBinThreads} * kHookBinPairs);
                 bin.add(i, nodes * bin_chunks, nodes, bin_chunks, desc_count, row_count, row_words,
@@ -592,7 +626,6 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
             desc_count += nodes;
             row_count += nodes * row_words;
         }
-        lp.hbm = hbm.commit(table);
         // First hooks are binned where the direct stores are bound by address
         // translation: nodes of at least 64 MB of 32-bit words in a level of
         // at least 2^31 entries (measured at 2^26 and 2^32 + 12345 entries:
@@ -600,14 +633,54 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         // nodes hold 64 MB or more).  The test setting bins every node whose
         // words overflow an XCD's L2.
         const uint64_t node_bytes = lp.max_slots * 4;
-        lp.binned = lp.hbm.count && !lp.wide_uf &&
+        lp.binned = !hbm_items.empty() && !lp.wide_uf &&
                     (hook_binning == 1 ? node_bytes > (uint64_t{4} << 20)
                      : hook_binning < 0 ? node_bytes >= (uint64_t{64} << 20) &&
                                               total_n >= (uint64_t{1} << 31)
                                         : false);
+        // Many small nodes of several items (a batch): lane groups, so that
+        // each node keeps to one XCD at every depth, the shallow ones
+        // included, and each XCD interleaves about kPlainUnionGroupBytes of
+        // nodes of several items.  One item, or few nodes, keep the node
+        // mappings of block_to_node.
+        const uint64_t hbm_nodes = hbm_items.size() * nodes;
+        lp.lanes = hbm_items.size() >= 2 && hbm_nodes >= 64 && hbm_nodes <= (uint64_t{1} << 16) &&
+                   !lp.wide_uf && !lp.binned && !rulers && node_bytes <= kPlainUnionMaxBytes;
+        if (lp.lanes) {
+            // Largest nodes first, so that a group's lanes have about the
+            // same number of chunks.
+            std::stable_sort(hbm_items.begin(), hbm_items.end(),
+                             [](const HbmItem& a, const HbmItem& b) { return a.chunks > b.chunks; });
+            std::vector<SegList::Lane> group;
+            uint32_t lane_g = 1;
+            for (const HbmItem& h : hbm_items) {
+                for (uint64_t j = 0; j < nodes; ++j) {
+                    if (group.empty()) {
+                        // Nodes of ~kPlainUnionGroupBytes of words per XCD.
+                        const uint64_t bytes = h.chunks * kBlock * 4;
+                        lane_g = 1;
+                        while (lane_g < 64 && bytes * lane_g * 2 <= kPlainUnionGroupBytes) lane_g *= 2;
+                    }
+                    group.push_back(SegList::Lane{h.item, h.chunks, h.desc0 + j,
+                                                  h.row0 + j * h.row_words, h.row_words});
+                    if (group.size() == uint64_t{8} * lane_g) {
+                        hbm.add_lane_group(group, lane_g);
+                        group.clear();
+                    }
+                }
+            }
+            if (!group.empty()) hbm.add_lane_group(group, lane_g);
+        } else {
+            for (const HbmItem& h : hbm_items) {
+                hbm.add(h.item, nodes * h.chunks, nodes, h.chunks, h.desc0, h.row0, h.row_words,
+                        true);
+            }
+        }
+        lp.hbm = hbm.commit(table);
         // Plain-store hooks where every node keeps to one XCD (8 nodes or
-        // more per item) and, by default, its words fit in the XCD's L2.
-        lp.plain = lp.hbm.count && d >= 3 && !lp.wide_uf && !lp.binned && !rulers &&
+        // more per item, or lane groups) and, by default, its words fit in
+        // the XCD's L2.
+        lp.plain = lp.hbm.count && (d >= 3 || lp.lanes) && !lp.wide_uf && !lp.binned && !rulers &&
                    (plain_union == 1 ||
                     (plain_union < 0 && node_bytes <= kPlainUnionMaxBytes));
         while (lp.plain_group < 64 && node_bytes * lp.plain_group * 2 <= kPlainUnionGroupBytes) {
@@ -828,7 +901,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                 // XCDs.  Coherent walks pay only there, and only when chains
                 // of neighbouring slots make up more than 1/64 of the slots
                 // (counted in every eighth block).
-                const bool spread = d < 3;
+                const bool spread = d < 3 && !lp.lanes;
                 const unsigned long long chain_threshold =
                     spread ? lp.hbm_entries / 2 / 64 / 8 : ~0ull;
                 if constexpr (std::is_same_v<W, uint32_t>) {

@@ -96,18 +96,29 @@ struct Seg {
     uint32_t item;
     uint32_t depth;      // the depth a k_describe segment describes
     uint32_t side;       // the side a k_pack / k_pack_narrow segment packs
+    // Lane groups of a level launch (lanes > 0): this entry is one node of
+    // its item (nodes == 1, its descriptor at desc0), lane `lane` of `lanes`
+    // consecutive entries that share block0; lane l runs on XCD l % 8, each
+    // XCD taking its lanes lane_g at a time (level_block).
+    uint32_t lane;
+    uint32_t lanes;
+    uint32_t lane_g;
     uint32_t pad;
 };
 
-// Segment of block b: the last one whose first block is <= b.
-__device__ __forceinline__ const Seg& find_seg_at(const Seg* segs, uint32_t nsegs, uint64_t b) {
+// Index of the segment of block b: the last one whose first block is <= b.
+__device__ __forceinline__ uint32_t find_seg_index(const Seg* segs, uint32_t nsegs, uint64_t b) {
     uint32_t lo = 0, hi = nsegs - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
         if (segs[mid].block0 <= b) lo = mid;
         else hi = mid - 1;
     }
-    return segs[lo];
+    return lo;
+}
+
+__device__ __forceinline__ const Seg& find_seg_at(const Seg* segs, uint32_t nsegs, uint64_t b) {
+    return segs[find_seg_index(segs, nsegs, b)];
 }
 
 __device__ __forceinline__ const Seg& find_seg(const Seg* segs, uint32_t nsegs) {
@@ -290,9 +301,10 @@ __device__ __forceinline__ bool block_to_node(const Seg& sg, uint64_t& node, uin
 
 // Segments of 8 nodes or more: each XCD visits its nodes `group` at a time,
 // their blocks interleaved.
-__device__ __forceinline__ bool block_to_node_grouped(const Seg& sg, uint64_t& node,
-                                                      uint64_t& chunk, uint32_t group) {
-    const uint32_t b = static_cast<uint32_t>(blockIdx.x - sg.block0);
+__device__ __forceinline__ bool block_to_node_grouped_at(const Seg& sg, uint64_t block,
+                                                         uint64_t& node, uint64_t& chunk,
+                                                         uint32_t group) {
+    const uint32_t b = static_cast<uint32_t>(block - sg.block0);
     const uint32_t nodes = static_cast<uint32_t>(sg.nodes);
     const uint32_t chunks = static_cast<uint32_t>(sg.chunks);
     const uint32_t xcd = b % kXcds, local = b / kXcds, per_xcd = nodes / kXcds;
@@ -304,12 +316,31 @@ __device__ __forceinline__ bool block_to_node_grouped(const Seg& sg, uint64_t& n
     return n < nodes && c < chunks;
 }
 
-// block_to_node_grouped where `group` is nonzero (segments of 8 nodes or
-// more), else the node-by-node mapping of block_to_node.
-__device__ __forceinline__ bool block_to_node_g(const Seg& sg, uint64_t& node, uint64_t& chunk,
-                                                uint32_t group) {
-    return group ? block_to_node_grouped(sg, node, chunk, group)
-                 : block_to_node(sg, node, chunk, false);
+// Block b of a level launch (k_hook_first to k_route_outputs): its segment
+// and (node, chunk).  A lane-grouped segment is one node of an item; the
+// lanes of a group share its blocks, lane l on XCD l % 8, each XCD taking
+// its lanes lane_g at a time with their blocks interleaved, so that the
+// small nodes of many items each keep to one XCD.  A block past the lanes of
+// its group, or past its lane's chunks, has no work.  Other segments map by
+// block_to_node_grouped_at (a nonzero `group`) or block_to_node_at.
+__device__ __forceinline__ bool level_block(const Seg* segs, uint32_t nsegs, uint64_t b,
+                                            uint32_t group, bool spread, const Seg*& seg,
+                                            uint64_t& node, uint64_t& chunk) {
+    const uint32_t i = find_seg_index(segs, nsegs, b);
+    const Seg& s = segs[i];
+    if (s.lanes) {
+        const uint32_t rel = static_cast<uint32_t>(b - s.block0);
+        const uint32_t x = rel % kXcds, j = rel / kXcds;
+        const uint32_t l = (j % s.lane_g) * kXcds + x;
+        if (l >= s.lanes) return false;
+        seg = segs + (i - s.lane + l);
+        node = 0;
+        chunk = j / s.lane_g;
+        return chunk < seg->chunks;
+    }
+    seg = &s;
+    if (group) return block_to_node_grouped_at(s, b, node, chunk, group);
+    return block_to_node_at(s, b, node, chunk, spread);
 }
 
 // Node descriptors of every depth above the fusion depth, in one launch:
@@ -400,10 +431,11 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
                                                        unsigned long long chain_threshold,
                                                        uint32_t part_shift, uint32_t part,
                                                        const int* __restrict__ errs) {
-    const Seg& sg = find_seg(segs, nsegs);
-    if (errs[sg.item]) return;
+    const Seg* sgp;
     uint64_t node, chunk;
-    if (!block_to_node(sg, node, chunk, false)) return;
+    if (!level_block(segs, nsegs, blockIdx.x, 0u, false, sgp, node, chunk)) return;
+    const Seg& sg = *sgp;
+    if (errs[sg.item]) return;
     const NodeDesc& nd = desc[sg.desc0 + node];
     const uint64_t off = nd.off, cnt = nd.cnt, h = cnt >> 1;
     const uint64_t q = chunk * kBlock + threadIdx.x;
@@ -773,10 +805,11 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
                                                   unsigned long long* __restrict__ deferred_count,
                                                   uint64_t defer_cap,
                                                   const int* __restrict__ errs) {
-    const Seg& sg = find_seg(segs, nsegs);
-    if (errs[sg.item]) return;
+    const Seg* sgp;
     uint64_t node, chunk;
-    if (!block_to_node(sg, node, chunk, true)) return;
+    if (!level_block(segs, nsegs, blockIdx.x, 0u, true, sgp, node, chunk)) return;
+    const Seg& sg = *sgp;
+    if (errs[sg.item]) return;
     const NodeDesc& nd = desc[sg.desc0 + node];
     const uint64_t off = nd.off, cnt = nd.cnt;
     const uint64_t q = chunk * kBlock + threadIdx.x;
@@ -833,10 +866,11 @@ __global__ __launch_bounds__(kBlock) void k_union_plain(const Seg* __restrict__ 
                                                         uint64_t defer_cap, uint32_t group,
                                                         const int* __restrict__ errs) {
     using W = uint32_t;
-    const Seg& sg = find_seg(segs, nsegs);
-    if (errs[sg.item]) return;
+    const Seg* sgp;
     uint64_t node, chunk;
-    if (!block_to_node_grouped(sg, node, chunk, group)) return;
+    if (!level_block(segs, nsegs, blockIdx.x, group, false, sgp, node, chunk)) return;
+    const Seg& sg = *sgp;
+    if (errs[sg.item]) return;
     const NodeDesc& nd = desc[sg.desc0 + node];
     const uint64_t off = nd.off, cnt = nd.cnt;
     const uint64_t q = chunk * kBlock + threadIdx.x;
@@ -894,10 +928,11 @@ __global__ __launch_bounds__(kVerifyThreads) void k_union_verify(
     uint64_t defer_cap, uint32_t group, unsigned long long* __restrict__ rejoined,
     const int* __restrict__ errs) {
     using W = uint32_t;
-    const Seg& sg = find_seg(segs, nsegs);
-    if (errs[sg.item]) return;
+    const Seg* sgp;
     uint64_t node, chunk;
-    if (!block_to_node_grouped(sg, node, chunk, group)) return;
+    if (!level_block(segs, nsegs, blockIdx.x, group, false, sgp, node, chunk)) return;
+    const Seg& sg = *sgp;
+    if (errs[sg.item]) return;
     const NodeDesc& nd = desc[sg.desc0 + node];
     const uint64_t off = nd.off, h = nd.cnt >> 1;
     W* const u = uf + off;
@@ -958,10 +993,11 @@ __global__ __launch_bounds__(kBlock) void k_flatten(const Seg* __restrict__ segs
     const unsigned long long total = counts[0];
     if (total == 0) return;
     for (uint64_t b = blockIdx.x; b < vblocks; b += gridDim.x) {
-        const Seg& sg = find_seg_at(segs, nsegs, b);
-        if (errs[sg.item]) continue;
+        const Seg* sgp;
         uint64_t node, chunk;
-        if (!block_to_node_at(sg, b, node, chunk, false)) continue;
+        if (!level_block(segs, nsegs, b, 0u, false, sgp, node, chunk)) continue;
+        const Seg& sg = *sgp;
+        if (errs[sg.item]) continue;
         const NodeDesc& nd = desc[sg.desc0 + node];
         const uint64_t cnt = nd.cnt;
         const uint64_t p = chunk * kBlock + threadIdx.x;
@@ -1019,10 +1055,11 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restri
                                                          unsigned long long chain_threshold,
                                                          uint32_t group,
                                                          const int* __restrict__ errs) {
-    const Seg& sg = find_seg(segs, nsegs);
-    if (errs[sg.item]) return;
+    const Seg* sgp;
     uint64_t node, chunk;
-    if (!block_to_node_g(sg, node, chunk, group)) return;
+    if (!level_block(segs, nsegs, blockIdx.x, group, false, sgp, node, chunk)) return;
+    const Seg& sg = *sgp;
+    if (errs[sg.item]) return;
     const ItemCtx it = items[sg.item];
     const NodeDesc& nd = desc[sg.desc0 + node];
     const uint64_t off = nd.off, cnt = nd.cnt, row = nd.row;
@@ -1075,10 +1112,11 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict_
                                                           void* __restrict__ next_uf,
                                                           uint32_t next_wide,
                                                           const int* __restrict__ errs) {
-    const Seg& sg = find_seg(segs, nsegs);
-    if (errs[sg.item]) return;
+    const Seg* sgp;
     uint64_t node, chunk;
-    if (!block_to_node(sg, node, chunk, false)) return;
+    if (!level_block(segs, nsegs, blockIdx.x, 0u, false, sgp, node, chunk)) return;
+    const Seg& sg = *sgp;
+    if (errs[sg.item]) return;
     const NodeDesc& nd = desc[sg.desc0 + node];
     const uint64_t off = nd.off, cnt = nd.cnt, row = nd.row;
     const uint64_t h = cnt >> 1, lc = h + (cnt & 1);
