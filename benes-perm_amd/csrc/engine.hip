@@ -343,6 +343,11 @@ struct Profiler {
     void count(const char* name, const unsigned long long* dev, size_t words) {
         if (on) counters.push_back(Counter{name, dev, words});
     }
+    // A host-side count, added to the table as launches.
+    std::vector<std::pair<const char*, uint64_t>> host_counts;
+    void note(const char* name, uint64_t n) {
+        if (on) host_counts.emplace_back(name, n);
+    }
     void begin(const char* name) {
         if (!on) return;
         Pending p{name, nullptr, nullptr};
@@ -387,6 +392,16 @@ struct Profiler {
             for (unsigned long long x : v) it->launches += x;
         }
         counters.clear();
+        for (const auto& c : host_counts) {
+            auto it = std::find_if(g_prof_stats.begin(), g_prof_stats.end(),
+                                   [&](const KernelStat& k) { return k.name == c.first; });
+            if (it == g_prof_stats.end()) {
+                g_prof_stats.push_back(KernelStat{c.first, 0, 0});
+                it = g_prof_stats.end() - 1;
+            }
+            it->launches += c.second;
+        }
+        host_counts.clear();
     }
 };
 
@@ -849,6 +864,9 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                   segp(describe_all), describe_all.count, ws.desc.as<NodeDesc>(), errs);
     }
 
+    for (const LevelPlan& lp : levels) {
+        if (lp.lanes) prof.note("lane_group_depths", 1);
+    }
     // Depth d reads level[(d - 1) & 1] and writes level[d & 1].
     for (uint32_t d = 0; d < max_df; ++d) {
         const LevelPlan& lp = levels[d];

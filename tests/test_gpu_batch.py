@@ -124,3 +124,42 @@ def test_batch_workload_shape():
     out = B.compress_batch(perms, fmts)
     for p, f, blob in zip(perms, fmts, out):
         assert blob == S.oracle_compress(p, f.middle_group_log2, 0)
+
+
+@pytest.mark.parametrize("plain", [-1, 0], ids=["plain-store", "cas"])
+def test_lane_groups_match_oracle(plain):
+    # Enough items of 2^17..2^18 entries that their HBM depths take lane
+    # groups (each small node on one XCD, several items' nodes interleaved
+    # per XCD, partial last groups, lanes of unequal chunk counts), with the
+    # union pass in either form; one bad item among them fails alone.
+    rng = np.random.default_rng(0x1A4E)
+    perms, fmts, args = [], [], []
+    for i in range(72):
+        n = int(2 ** rng.uniform(17, 18)) + int(rng.integers(0, 4))
+        kind = int(rng.integers(0, 3))
+        width = 8 if rng.random() < 0.25 else 4
+        perms.append(B.generate(kind, n, 7000 + i, 5000, width))
+        g, pl, ic, oc = random_format(rng, n)
+        fmts.append(S.fmt(g, pl, ic, oc))
+        args.append((g, pl, ic, oc))
+    bad = np.arange((1 << 17) + 5, dtype=np.uint32)
+    bad[77] = 78
+    perms.insert(30, bad)
+    fmts.insert(30, None)
+    args.insert(30, None)
+    B.set_plain_union(plain)
+    B.set_profiling(True)
+    try:
+        out = B.compress_batch(perms, fmts)
+        stats = B.kernel_stats()
+    finally:
+        B.set_profiling(False)
+        B.set_plain_union(-1)
+    assert stats.get("lane_group_depths", (0, 0))[1] >= 2, stats
+    assert (stats.get("k_union_plain", (0, 0))[1] > 0) == (plain == -1), stats
+    for p, a, blob in zip(perms, args, out):
+        if a is None:
+            assert isinstance(blob, ValueError)
+            continue
+        assert not isinstance(blob, Exception), blob
+        assert blob == S.oracle_compress(p, *a), (p.size, a)
