@@ -922,6 +922,11 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                 const bool spread = d < 3 && !lp.lanes;
                 const unsigned long long chain_threshold =
                     spread ? lp.hbm_entries / 2 / 64 / 8 : ~0ull;
+                // The first hooks (direct or binned) store only the words that
+                // get a parent: the others start as root words, set at the
+                // root here and below it by the previous depth's
+                // k_route_outputs.
+                if (d == 0) fill(uf, total_n * sizeof(W) / 4, 0xFFFFFFFFu);
                 if constexpr (std::is_same_v<W, uint32_t>) {
                     if (lp.binned) {
                         // Staged in this depth's output level arrays, which
@@ -943,10 +948,6 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                                   st, uf, pdst, idst, fill, lp.bins, lp.bin_log2);
                     }
                 }
-                // The first hooks store only the words that get a parent:
-                // the others start as root words, set at the root here and
-                // below it by the previous depth's k_route_outputs.
-                if (d == 0 && !lp.binned) fill(uf, total_n * sizeof(W) / 4, 0xFFFFFFFFu);
                 if (lp.binned) {
                     // Hooked above.
                 } else if (spread) {
@@ -1019,8 +1020,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                           srows, dcount + 2, chain_threshold, spread ? 0u : lp.plain_group,
                           errs);
                 // Root words for the next depth's first hooks.
-                const bool next_hooks = d + 1 < max_df && levels[d + 1].hbm.count &&
-                                        !levels[d + 1].binned;
+                const bool next_hooks = d + 1 < max_df && levels[d + 1].hbm.count;
                 BH_LAUNCH(prof, sizeof(VI) == 8 ? "k_route_outputs_w64" : "k_route_outputs",
                           (k_route_outputs<VI>), grid(lp.hbm),
                           dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, idst, desc, srows,

@@ -473,7 +473,8 @@ __global__ __launch_bounds__(kBlock) void k_hook_first(const Seg* __restrict__ s
 // in LDS, reserves a run in each bin with one atomic, and writes the runs out
 // contiguously.  k_hook_scatter then writes the bins, each by the blocks of
 // one XCD, so that a bin's words stay within one XCD's L2 and translation
-// caches while they are written.  Every slot is written once, so bin b
+// caches while they are written.  Only the slots that get a parent are
+// written (the others start as root words), each once, so bin b
 // receives at most 2^bin_log2 stores and its run fits in positions
 // [b << bin_log2, (b + 1) << bin_log2) of the staging arrays, which are the
 // level arrays this depth writes only after its union-find is done.
@@ -527,26 +528,19 @@ __global__ __launch_bounds__(kHookBinThreads) void k_hook_bin(
                 static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q + 1));
             const uint64_t sa = in_slot(a, cnt), sb = in_slot(b, cnt);
             const uint32_t label = static_cast<uint32_t>(1 ^ ((a ^ b) & 1));
-            if (!(a & 1)) {
+            // Only the words that get a parent (the others start as root
+            // words, as for k_hook_first).
+            if (!(a & 1) && sb < sa) {
                 has[2 * k] = true;
                 rel[2 * k] = static_cast<uint32_t>(rel0 + sa);
-                val[2 * k] = sb < sa ? (static_cast<uint32_t>(sb) << 1) | label
-                                     : UfWord<uint32_t>::root;
+                val[2 * k] = (static_cast<uint32_t>(sb) << 1) | label;
             }
-            if (!(b & 1)) {
+            if (!(b & 1) && sa < sb) {
                 has[2 * k + 1] = true;
                 rel[2 * k + 1] = static_cast<uint32_t>(rel0 + sb);
-                val[2 * k + 1] = sa < sb ? (static_cast<uint32_t>(sa) << 1) | label
-                                         : UfWord<uint32_t>::root;
+                val[2 * k + 1] = (static_cast<uint32_t>(sa) << 1) | label;
             }
             next_to = (!(a & 1) && sb + 1 == sa) || (!(b & 1) && sa + 1 == sb);
-        } else if ((cnt & 1) && q == h) {
-            const uint64_t x = static_cast<uint64_t>(inv[off + cnt - 1]);
-            if (!(x & 1)) {
-                has[2 * k] = true;
-                rel[2 * k] = static_cast<uint32_t>(rel0 + in_slot(x, cnt));
-                val[2 * k] = UfWord<uint32_t>::root;
-            }
         }
         if (kCount) adjacent_hooks += static_cast<uint32_t>(__popcll(__ballot(next_to)));
     }
