@@ -2190,11 +2190,13 @@ __global__ __launch_bounds__(Threads) __attribute__((amdgpu_waves_per_eu(
         }
         __syncthreads();
         // Pair-indexed steps run over a flat index g = node * S + pair, S the
-        // power of two covering a node's pairs (and the odd tail slot when
-        // some node is odd): a wave covers 64 / S small nodes at once, or 64
-        // pairs of one large node.
-        const uint32_t odd_any = (q & 1u) | (m != 0 ? 1u : 0u);
-        const uint32_t s_log2 = 32u - __builtin_clz(((cmax >> 1) + odd_any - 1) | 1u);
+        // power of two covering a node's pairs and, in an odd node, its tail
+        // slot (index c >> 1): (c + 1) >> 1 indices for c entries, at most
+        // (cmax + 1) >> 1.  A wave covers 64 / S small nodes at once, or 64
+        // pairs of one large node.  (Sizing S by the largest node's pairs
+        // plus one whenever any node is odd doubled it below most subtrees
+        // of sizes that are not powers of two, such as 4095 = 2048 + 2047.)
+        const uint32_t s_log2 = 32u - __builtin_clz((((cmax + 1) >> 1) - 1) | 1u);
         const uint32_t S = 1u << s_log2, total = nodes << s_log2;
         // Union of the two input slots reaching each output pair (the same
         // input-slot forest as the HBM levels).  Each lane takes kUnionBatch
