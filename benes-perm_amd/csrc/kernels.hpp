@@ -1995,16 +1995,36 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
                 uint32_t f = first;
                 while (f < end && sub_count(n0, k, f) <= L) ++f;
                 if (f == end) continue;
+                // The first node's address is at hand when it is the
+                // level's first node (its walk from the subtree root is a
+                // chain of dependent table reads).
                 const uint64_t path = (b0 << k) | f;
                 const SideAddr a =
-                    address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
-                uint64_t dst = secbase + bit_of(a, first_pair);
+                    f == 0 ? first_addr[side][k]
+                           : address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d,
+                                        path);
+                // The run (at most 64 bits) is gathered in two words, then
+                // ORed into the blob: two atomics per run, not one per node.
+                const uint64_t dst0 = secbase + bit_of(a, first_pair);
+                uint32_t at = static_cast<uint32_t>(dst0 & 63);
+                uint64_t w0 = 0, w1 = 0;
                 for (uint32_t pi = f; pi < end; ++pi) {
                     const uint32_t share = static_cast<uint32_t>(
                         packet_share(sub_count(n0, k, pi), pk, span_log2, L));
-                    if (share) or_word(blob, dst, lds_bits(rows, noff[pi] + first_pair, share));
-                    dst += share;
+                    if (share) {
+                        const uint64_t v = lds_bits(rows, noff[pi] + first_pair, share);
+                        if (at < 64) {
+                            w0 |= v << at;
+                            if (at + share > 64) w1 |= v >> (64 - at);
+                        } else {
+                            w1 |= v << (at - 64);
+                        }
+                    }
+                    at += share;
                 }
+                auto* const w = reinterpret_cast<unsigned long long*>(blob) + (dst0 >> 6);
+                if (w0) atomicOr(w, static_cast<unsigned long long>(w0));
+                if (w1) atomicOr(w + 1, static_cast<unsigned long long>(w1));
             }
             continue;
         }
@@ -2020,7 +2040,8 @@ __device__ __forceinline__ void emit_level_runs(const Plan& P, const SubtreeRoot
             if (f == end) continue;
             const uint64_t path = (b0 << k) | f;
             const SideAddr a =
-                address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
+                f == 0 ? first_addr[side][k]
+                       : address_at(P, walk_from(P, side, root.walk[side], d0, d, path), d, path);
             const uint64_t run0 = secbase + bit_of(a, first_pair);
             uint64_t carry = 0;
             for (uint32_t base = f; base < end; base += 64) {
