@@ -80,11 +80,12 @@ def algorithmic_bytes(kernel: str, n: int, blob: int) -> float:
 
 
 def union_pass(stats: dict, prof_steps: int) -> dict:
-    ms = sum(stats.get(k, (0.0, 0))[0] for k in ("k_union", "k_union_plain", "k_union_verify"))
+    kernels = ["k_union", "k_union_plain", "k_union_verify", "k_union_rejoin"]
+    ms = sum(stats.get(k, (0.0, 0))[0] for k in kernels)
     depths = (stats.get("k_union", (0, 0))[1] + stats.get("k_union_plain", (0, 0))[1]) / prof_steps
     per_depth = ms / prof_steps / depths if depths else 0.0
     achieved = 4.0 * N / (per_depth * 1e-3) / 1e9 if per_depth else 0.0
-    return {"kernels": ["k_union", "k_union_plain", "k_union_verify"],
+    return {"kernels": kernels,
             "ms_per_step": ms / prof_steps, "depths": depths, "ms_per_depth": per_depth,
             "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS}
 
@@ -613,10 +614,10 @@ def main():
         },
         "kernel_ms_per_step": {k: v[0] / prof_steps for k, v in sorted(stats.items())},
         # Union relations overwritten by a concurrent plain-store hook and
-        # joined again by k_union_verify, per compress.
+        # joined again by k_union_rejoin, per compress.
         "union_rejoined_per_step": all_stats.get("union_rejoined", (0, 0))[1] / prof_steps,
         # The union pass of every HBM depth together (compare-and-swap
-        # launches, plain-store launches and their checks): per depth, against
+        # launches, plain-store launches, their checks and re-joins): per depth, against
         # its algorithmic bytes (the inverse read, 4 B/entry).
         "union_pass": union_pass(stats, prof_steps),
         "pipeline_roofline": {

@@ -150,7 +150,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     DevBuf input, level[2], ilevel[2], inv0, uf, srows, trows, desc, plans, items, segs, errs,
-        blob, deferred, deferred_counts, bin_fill, rulers, lbits;
+        blob, deferred, deferred_counts, bin_fill, rulers, lbits, lost_counts;
     std::vector<int> host_errs;
     PinnedRing ring;  // host <-> device staging of this workspace's stream
     Workspace() = default;
@@ -176,7 +176,7 @@ struct Workspace {
     void bind(Arena* a) {
         for (DevBuf* b : {&input, &level[0], &level[1], &ilevel[0], &ilevel[1], &inv0, &uf, &srows,
                           &trows, &desc, &plans, &items, &segs, &errs, &blob, &deferred,
-                          &deferred_counts, &bin_fill, &rulers, &lbits}) {
+                          &deferred_counts, &bin_fill, &rulers, &lbits, &lost_counts}) {
             b->release();
             b->arena = a;
         }
@@ -794,6 +794,12 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         // counters of the relations k_union_verify joined again (counted
         // when profiling).
         ws.deferred_counts.ensure(sizeof(unsigned long long) * (4 + kRejoinCounters) * max_df);
+        // One count per check block of the plain-store union pass.
+        uint64_t check_blocks = 0;
+        for (const LevelPlan& lp : levels) {
+            if (lp.plain) check_blocks = std::max<uint64_t>(check_blocks, lp.hbm.blocks);
+        }
+        if (check_blocks) ws.lost_counts.ensure(sizeof(uint32_t) * check_blocks);
         if (fill_words) ws.bin_fill.ensure(sizeof(uint32_t) * fill_words);
         if (rulers) ws.rulers.ensure((sizeof(uint64_t) + sizeof(uint32_t)) * 2 * ruler_buckets);
     }
@@ -990,10 +996,18 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                         BH_LAUNCH(prof, "k_union_plain", (k_union_plain<VI>), grid(lp.hbm),
                                   dim3(kBlock), 0, st, segp(lp.hbm), lp.hbm.count, inv, uf, desc,
                                   rec, deferred, dcount, defer_cap, lp.plain_group, errs);
+                        uint32_t* const lost = ws.lost_counts.as<uint32_t>();
                         BH_LAUNCH(prof, "k_union_verify", (k_union_verify<VI>), grid(lp.hbm),
                                   dim3(kVerifyThreads), 0, st, segp(lp.hbm), lp.hbm.count, uf,
-                                  desc, rec, deferred, dcount, defer_cap, lp.plain_group,
-                                  prof.on ? rejoined : nullptr, errs);
+                                  desc, rec, lost, lp.plain_group, errs);
+                        // kRejoinSpan check blocks of one XCD per block.
+                        const uint64_t per_xcd = ceil_div(lp.hbm.blocks, uint64_t{kXcds});
+                        const uint64_t rejoin_blocks = kXcds * ceil_div(per_xcd, uint64_t{kRejoinSpan});
+                        BH_LAUNCH(prof, "k_union_rejoin", k_union_rejoin,
+                                  dim3(static_cast<uint32_t>(rejoin_blocks)), dim3(kBlock), 0, st,
+                                  segp(lp.hbm), lp.hbm.count, uf, desc, rec, lost, lp.hbm.blocks,
+                                  lp.plain_group, deferred, dcount, defer_cap,
+                                  prof.on ? rejoined : nullptr);
                         prof.count("union_rejoined", rejoined, kRejoinCounters);
                         plain_done = true;
                     }
@@ -1415,7 +1429,7 @@ constexpr uint64_t kNoAllocDeferCap = uint64_t{1} << 14;
 // Worst case at 2^20 entries (the smallest subtree shape, 64-bit words
 // forced): 8 MiB inverse, four 4 MiB level arrays, 8 MiB of union-find
 // words, 384 KiB of deferred edges, under 300 KiB of rows, descriptors,
-// segments and plans.
+// segments, plans and the union check's counts.
 static_assert(kNoAllocScratchBytes >= 33 * (uint64_t{1} << 20) &&
                   kNoAllocMaxEntries == (uint64_t{1} << 20),
               "scratch sized for 2^20 entries");

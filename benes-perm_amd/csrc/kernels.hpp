@@ -23,12 +23,12 @@
 //                    one thread per output pair: parity union-find over the
 //                    node's input slots in HBM (plain-store first hooks, then
 //                    one CAS per remaining hook; long walks deferred)
-//   k_union_plain, k_union_verify
+//   k_union_plain, k_union_verify, k_union_rejoin
 //                    instead of k_union where a node keeps to one XCD and its
 //                    words fit in 8 MB: the hooks are plain stores, each XCD
 //                    uniting about 2 MB of nodes at a time in its L2; the
-//                    check then joins again every relation a concurrent hook
-//                    overwrote
+//                    check then collects every relation a concurrent hook
+//                    overwrote, and the re-join joins them again
 //   k_hook_bin, k_hook_scatter
 //                    the first hooks of very large nodes instead: staged by
 //                    word bin, then written bin by bin, each bin by one XCD
@@ -902,39 +902,46 @@ __global__ __launch_bounds__(kBlock) void k_union_plain(const Seg* __restrict__ 
     __builtin_nontemporal_store(r, rec + ((off + 2 * q) >> 1));
 }
 
-// After k_union_plain: each recorded hook is checked against its word, and a
-// relation that a later hook overwrote is joined again, with
-// compare-and-swap hooks (the slot has some smaller parent by now, so the
-// relation is recorded between the two trees).  The blocks take the nodes in
-// the order k_union_plain did, so each node's words are read while in L2.
-// One wave per block of kBlock pairs: each lane issues the loads of its four
-// records, then of the four words they name, before comparing any, so that
-// the wave waits for memory twice instead of eight times.
+// After k_union_plain: each recorded hook is checked against its word.  A
+// relation that a later hook overwrote must be joined again; the check only
+// collects them, and k_union_rejoin joins them, so that the check's waves do
+// not wait on the walks (about one relation in 70 is lost, nearly one per
+// wave).  The blocks take the nodes in the order k_union_plain did, so each
+// node's words are read while in L2.  One wave per block of kBlock pairs:
+// each lane issues the loads of its four records, then of the four words
+// they name, before comparing any, so that the wave waits for memory twice
+// instead of eight times.  The lost records of a block are then written,
+// packed, over the block's own records (all of which the wave has read by
+// then), and their count to lost_counts[block] (0 for a block with no work).
 constexpr uint32_t kVerifyThreads = 64;
 constexpr uint32_t kRejoinCounters = 1024;
 constexpr uint32_t kVerifyPairs = kBlock / kVerifyThreads;
 
 template <typename VI>
 __global__ __launch_bounds__(kVerifyThreads) void k_union_verify(
-    const Seg* __restrict__ segs, uint32_t nsegs, uint32_t* __restrict__ uf,
-    const NodeDesc* __restrict__ desc, const uint64_t* __restrict__ rec,
-    DeferredEdge* __restrict__ deferred, unsigned long long* __restrict__ deferred_count,
-    uint64_t defer_cap, uint32_t group, unsigned long long* __restrict__ rejoined,
-    const int* __restrict__ errs) {
+    const Seg* __restrict__ segs, uint32_t nsegs, const uint32_t* __restrict__ uf,
+    const NodeDesc* __restrict__ desc, uint64_t* __restrict__ rec,
+    uint32_t* __restrict__ lost_counts, uint32_t group, const int* __restrict__ errs) {
     using W = uint32_t;
     const Seg* sgp;
     uint64_t node, chunk;
-    if (!level_block(segs, nsegs, blockIdx.x, group, false, sgp, node, chunk)) return;
-    const Seg& sg = *sgp;
-    if (errs[sg.item]) return;
-    const NodeDesc& nd = desc[sg.desc0 + node];
+    if (!level_block(segs, nsegs, blockIdx.x, group, false, sgp, node, chunk) ||
+        errs[sgp->item]) {
+        if (threadIdx.x == 0) lost_counts[blockIdx.x] = 0;
+        return;
+    }
+    const NodeDesc& nd = desc[sgp->desc0 + node];
     const uint64_t off = nd.off, h = nd.cnt >> 1;
-    W* const u = uf + off;
+    const W* const u = uf + off;
+    // Record of pair q: rec[(off + 2q) >> 1]; the block's pairs are
+    // consecutive, from rec[base].
+    const uint64_t base = (off + 2 * chunk * kBlock) >> 1;
     uint64_t r[kVerifyPairs];
 #pragma unroll
     for (uint32_t k = 0; k < kVerifyPairs; ++k) {
         const uint64_t q = chunk * kBlock + k * kVerifyThreads + threadIdx.x;
-        r[k] = q < h ? __builtin_nontemporal_load(rec + ((off + 2 * q) >> 1)) : kNoHook;
+        r[k] = q < h ? __builtin_nontemporal_load(rec + base + k * kVerifyThreads + threadIdx.x)
+                     : kNoHook;
     }
     W w[kVerifyPairs];
 #pragma unroll
@@ -944,25 +951,75 @@ __global__ __launch_bounds__(kVerifyThreads) void k_union_verify(
     uint32_t lost = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kVerifyPairs; ++k) {
-        if (r[k] == kNoHook || w[k] == static_cast<W>(r[k])) continue;
-        ++lost;
-        const W x = static_cast<W>(r[k] >> 32), word = static_cast<W>(r[k]);
+        lost += r[k] != kNoHook && w[k] != static_cast<W>(r[k]);
+    }
+    const uint32_t incl = static_cast<uint32_t>(wave_inclusive_sum(lost));
+    uint64_t at = base + incl - lost;
+#pragma unroll
+    for (uint32_t k = 0; k < kVerifyPairs; ++k) {
+        if (r[k] != kNoHook && w[k] != static_cast<W>(r[k])) rec[at++] = r[k];
+    }
+    if (threadIdx.x == 63) lost_counts[blockIdx.x] = incl;
+}
+
+// Joins again the relations k_union_verify collected: x -> (y, p) is joined
+// with compare-and-swap hooks (the slot has some smaller parent by now, so the
+// relation is made between the two trees).  Each block takes the counts of
+// kRejoinSpan check blocks of one XCD (check block b ran on XCD b % 8) and
+// spreads their relations over its threads, one each, so that the walks run
+// side by side in full waves.
+constexpr uint32_t kRejoinSpan = 64;
+
+__global__ __launch_bounds__(kBlock) void k_union_rejoin(
+    const Seg* __restrict__ segs, uint32_t nsegs, uint32_t* __restrict__ uf,
+    const NodeDesc* __restrict__ desc, const uint64_t* __restrict__ rec,
+    const uint32_t* __restrict__ lost_counts, uint64_t vblocks, uint32_t group,
+    DeferredEdge* __restrict__ deferred, unsigned long long* __restrict__ deferred_count,
+    uint64_t defer_cap, unsigned long long* __restrict__ rejoined) {
+    using W = uint32_t;
+    __shared__ uint32_t start[kRejoinSpan + 1];
+    // Check blocks x + kXcds * (kRejoinSpan * g + i), i < kRejoinSpan, for
+    // this block's XCD x and group g.
+    const uint64_t x = blockIdx.x % kXcds, g = blockIdx.x / kXcds;
+    auto check_block = [&](uint32_t i) { return x + kXcds * (kRejoinSpan * g + i); };
+    if (threadIdx.x < 64) {
+        const uint64_t b = check_block(threadIdx.x);
+        const uint32_t c = b < vblocks ? lost_counts[b] : 0u;
+        start[threadIdx.x + 1] = static_cast<uint32_t>(wave_inclusive_sum(c));
+        if (threadIdx.x == 0) start[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t total = start[kRejoinSpan];
+    for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
+        // The check block whose relations hold t: start[i] <= t < start[i + 1].
+        uint32_t lo = 0, hi = kRejoinSpan;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (start[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const Seg* sgp;
+        uint64_t node, chunk;
+        // A check block that counted relations maps to a node.
+        if (!level_block(segs, nsegs, check_block(lo), group, false, sgp, node, chunk)) continue;
+        const NodeDesc& nd = desc[sgp->desc0 + node];
+        const uint64_t off = nd.off;
+        const uint64_t r = rec[((off + 2 * chunk * kBlock) >> 1) + (t - start[lo])];
+        W* const u = uf + off;
+        const W xs = static_cast<W>(r >> 32), word = static_cast<W>(r);
         // value(x) ^ value(y) == p for the relation x -> (y, p).
-        if (uf_union_rem<DeviceForest>(u, x, 0u, word >> 1, 0u, word & 1u, kWalkCap)) continue;
+        if (uf_union_rem<DeviceForest>(u, xs, 0u, word >> 1, 0u, word & 1u, kWalkCap)) continue;
         const unsigned long long i = atomicAdd(deferred_count, 1ull);
         if (i < defer_cap) {
-            deferred[i] = DeferredEdge{off, (word >> 1) | (static_cast<uint64_t>(word & 1u) << 63), x};
+            deferred[i] = DeferredEdge{off, (word >> 1) | (static_cast<uint64_t>(word & 1u) << 63), xs};
         } else {
-            uf_union_rem<DeviceForest>(u, x, 0u, word >> 1, 0u, word & 1u, ~uint64_t{0});
+            uf_union_rem<DeviceForest>(u, xs, 0u, word >> 1, 0u, word & 1u, ~uint64_t{0});
         }
     }
-    if (rejoined) {
-        // Counted when profiling, into one of kRejoinCounters counters per
-        // depth, so that few waves share one.
-        const unsigned long long total = wave_inclusive_sum(lost);
-        if (lane_id() == 63 && total) {
-            atomicAdd(rejoined + (blockIdx.x & (kRejoinCounters - 1)), total);
-        }
+    // Counted when profiling, into one of kRejoinCounters counters per depth.
+    if (rejoined && threadIdx.x == 0 && total) {
+        atomicAdd(rejoined + (blockIdx.x & (kRejoinCounters - 1)),
+                  static_cast<unsigned long long>(total));
     }
 }
 

@@ -1,8 +1,8 @@
 # SPDX-License-Identifier: Apache-2.0
 """The union pass of the depths whose nodes keep to one XCD, in both of its
 forms: compare-and-swap hooks (k_union) and plain-store hooks checked and
-rejoined afterwards (k_union_plain + k_union_verify), each forced at every
-such depth with benes_hip_set_plain_union.  By default the plain form runs
+rejoined afterwards (k_union_plain, k_union_verify, k_union_rejoin), each
+forced at every such depth with benes_hip_set_plain_union.  By default the plain form runs
 only where a node's words fit in an XCD's L2.  Both must give the canonical
 colouring of the reference (src/benes.cpp:247-297), so every blob is checked
 byte for byte against the oracle or a digest of the reference library's own
@@ -53,14 +53,16 @@ def local_depths(n: int) -> int:
 
 def assert_union_kernels(stats, mode, local_depths):
     """With `local_depths` HBM depths whose nodes keep to one XCD, mode 1 runs
-    k_union_plain + k_union_verify there and mode 0 never does."""
+    k_union_plain, k_union_verify and k_union_rejoin there and mode 0 never
+    does."""
     plain = stats.get("k_union_plain", (0, 0))[1]
     verify = stats.get("k_union_verify", (0, 0))[1]
+    rejoin = stats.get("k_union_rejoin", (0, 0))[1]
     if mode == 1:
-        assert plain == local_depths and verify == local_depths, stats
+        assert plain == local_depths and verify == local_depths and rejoin == local_depths, stats
         print(f"  relations joined again: {stats.get('union_rejoined', (0, 0))[1]}", flush=True)
     else:
-        assert plain == 0 and verify == 0, stats
+        assert plain == 0 and verify == 0 and rejoin == 0, stats
 
 
 @pytest.mark.parametrize("n,g,pl,ic,oc,width", [
