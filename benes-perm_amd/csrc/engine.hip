@@ -570,6 +570,18 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         per_entry_list.add(i, ceil_div(jobs[i].n, kBlock * per_thread), 0, 0, 0, 0, 0, false);
     }
     const Launch per_entry = per_entry_list.commit(table);
+    // k_check_inverse streams kCheckPerThread entries per thread: its blocks
+    // do one load each otherwise, and the segment search that starts every
+    // block costs more than that in a batch of many items (4096 items: 3.1 ->
+    // 0.8 ms).  (k_scatter_inverse keeps one entry per thread: a loop of
+    // dependent load-store pairs per thread was slower, 5.4 -> 7.2 ms at 8.)
+    constexpr uint64_t kCheckPerThread = 16;
+    SegList check_list;
+    for (uint32_t i = 0; i < count; ++i) {
+        check_list.add(i, ceil_div(jobs[i].n, kBlock * per_thread * kCheckPerThread), 0, 0, 0, 0,
+                       0, false);
+    }
+    const Launch check_entry = check_list.commit(table);
     struct LevelPlan {
         Launch hbm, lds, pack, narrow;  // pack, narrow: both sides' segments
         Launch bin;                     // k_hook_bin: the HBM nodes, 4096 pairs per block
@@ -860,8 +872,8 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                       segp(per_entry), per_entry.count, inv, errs);
         }
         BH_LAUNCH(prof, w64 ? "k_check_inverse_w64" : "k_check_inverse", (k_check_inverse<VI>),
-                  grid(per_entry), dim3(kBlock), 0, st, ditems, segp(per_entry), per_entry.count,
-                  inv, errs);
+                  grid(check_entry), dim3(kBlock), 0, st, ditems, segp(check_entry),
+                  check_entry.count, inv, errs);
     };
     hipLaunchKernelGGL(k_headers, dim3(ceil_div(count, 64)), dim3(64), 0, st, ditems, count,
                        blob_ptr);
