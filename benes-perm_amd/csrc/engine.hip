@@ -645,11 +645,15 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                          ceil_div(groups * packets * pack_words_per_job(sd.group_depth), kBlock),
                          nodes, packets, desc_count, row_count, row_words, false, d, side);
                 if (pack_narrow(sd.group_depth, span_log2)) {
-                    // One block per (group, kNarrowGroups node groups, row word).
+                    // One block per (group, kNarrowGroups (node group, row
+                    // word) pairs): narrow_shape.
                     const uint64_t node_groups =
                         (uint64_t{1} << sd.relative_level) >> (6 - span_log2);
-                    narrow.add(i, groups * ceil_div(node_groups, kNarrowGroups) * row_words,
-                               nodes, 0, desc_count, row_count, row_words, false, d, side);
+                    const NarrowShape sh = narrow_shape(node_groups);
+                    if (sh.ngb) {
+                        narrow.add(i, groups * sh.chunks * ceil_div(row_words, sh.wpb), nodes, 0,
+                                   desc_count, row_count, row_words, false, d, side);
+                    }
                 }
             }
             desc_count += nodes;

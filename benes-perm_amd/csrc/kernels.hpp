@@ -1646,16 +1646,28 @@ __global__ __launch_bounds__(kBlock) void k_pack(const ItemCtx* __restrict__ ite
 // Full-share packets of the narrow levels (pack_narrow).  A packet's run is
 // the concatenation, over the group's nodes in path order, of each node's
 // span bits [first_pair, first_pair + span): the run's word k holds the
-// 64/span nodes of node group k.  A block takes (set, kNarrowGroups node
-// groups, one row word = 64 pairs), one wave per node group: lane L loads
-// the row word of the group's node L / span, and one ballot per packet of
-// those 64 pairs, over the bits (packet * span + L % span), is that
-// packet's word for the group.  One load per lane gives 64/span words, where
-// gathering each word from the rows costs 64/span loads of its own.  The
-// words go through LDS so that consecutive threads store consecutive words
-// of one run.
+// 64/span nodes of node group k.  A block holds kNarrowGroups waves, one per
+// (node group, row word = 64 pairs): lane L loads the row word of the group's
+// node L / span, and one ballot per packet of those 64 pairs, over the bits
+// (packet * span + L % span), is that packet's word for the group.  One load
+// per lane gives 64/span words, where gathering each word from the rows costs
+// 64/span loads of its own.  The words go through LDS so that consecutive
+// threads store consecutive words of one run.  A group-root set with fewer
+// than kNarrowGroups node groups (small items) gives its block's remaining
+// waves further row words instead of leaving them idle (narrow_shape).
 constexpr uint32_t kNarrowGroups = 16;
 constexpr uint32_t kNarrowThreads = 64 * kNarrowGroups;
+
+// A narrow block's shape for `node_groups` node groups per set (a power of
+// two): `ngb` node groups of each of `wpb` row words, `chunks` blocks across
+// the node groups.
+struct NarrowShape {
+    uint64_t ngb, wpb, chunks;
+};
+__host__ __device__ __forceinline__ NarrowShape narrow_shape(uint64_t node_groups) {
+    const uint64_t ngb = node_groups < kNarrowGroups ? node_groups : kNarrowGroups;
+    return NarrowShape{ngb, ngb ? kNarrowGroups / ngb : 0, ngb ? node_groups / ngb : 0};
+}
 
 __global__ __launch_bounds__(kNarrowThreads) void k_pack_narrow(
     const ItemCtx* __restrict__ items, const Seg* __restrict__ segs, uint32_t nsegs, uint32_t d,
@@ -1676,22 +1688,25 @@ __global__ __launch_bounds__(kNarrowThreads) void k_pack_narrow(
     const uint32_t span = 1u << span_log2;
     const uint32_t per_word = 64u >> span_log2;  // nodes of a group; packets of a row word
     const uint64_t node_groups = (uint64_t{1} << r) >> (6 - span_log2);
-    const uint64_t chunks = (node_groups + kNarrowGroups - 1) / kNarrowGroups;
+    const NarrowShape sh = narrow_shape(node_groups);
+    if (sh.ngb == 0) return;
     const uint64_t rw = sg.row_words;
+    const uint64_t wblocks = (rw + sh.wpb - 1) / sh.wpb;
     const uint64_t b = blockIdx.x - sg.block0;
     const uint32_t root_depth = d - r;
-    const uint64_t set = b / (chunks * rw);
+    const uint64_t set = b / (sh.chunks * wblocks);
     if (set >= (uint64_t{1} << root_depth)) return;
-    const uint64_t rest = b - set * (chunks * rw);
-    const uint64_t chunk = rest / rw, w = rest - chunk * rw;
+    const uint64_t rest = b - set * (sh.chunks * wblocks);
+    const uint64_t chunk = rest / wblocks, wb = rest - chunk * wblocks;
     const uint64_t C = node_count(P.n, root_depth, set);
     if ((C >> r) <= P.leaf) return;
     const uint64_t min_pairs = (C >> r) >> 1;
-    if ((w << 6) >= min_pairs) return;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint64_t b0 = set << r;
-    const uint64_t ng = chunk * kNarrowGroups + wave;
-    if (ng < node_groups) {
+    // Wave v: node group chunk * ngb + v % ngb of row word wb * wpb + v / ngb.
+    const uint64_t w = wb * sh.wpb + wave / sh.ngb;
+    const uint64_t ng = chunk * sh.ngb + wave % sh.ngb;
+    if (w < rw && (w << 6) < min_pairs) {
         const uint64_t node = b0 + ng * per_word + (lane >> span_log2);
         const uint64_t word = rows[sg.row0 + node * rw + w];
         for (uint32_t pk = 0; pk < per_word; ++pk) {
@@ -1700,14 +1715,16 @@ __global__ __launch_bounds__(kNarrowThreads) void k_pack_narrow(
         }
     }
     __syncthreads();
-    const uint32_t pk = threadIdx.x / kNarrowGroups, gl = threadIdx.x % kNarrowGroups;
-    const uint64_t ngs = chunk * kNarrowGroups + gl;
-    if (pk >= per_word || ngs >= node_groups) return;
-    const uint64_t first_pair = ((w * per_word) + pk) << span_log2;
+    // Thread (pk, column): packet pk's word of the column's wave.
+    const uint32_t pk = threadIdx.x / kNarrowGroups, col = threadIdx.x % kNarrowGroups;
+    const uint64_t wc = wb * sh.wpb + col / sh.ngb;
+    const uint64_t ngc = chunk * sh.ngb + col % sh.ngb;
+    if (pk >= per_word || wc >= rw) return;
+    const uint64_t first_pair = ((wc * per_word) + pk) << span_log2;
     if (first_pair + span > min_pairs) return;  // not a full share: k_pack places it
     const uint64_t secbase = (side == 0 ? P.input_offset : P.output_offset) * 8;
-    const uint64_t dst = secbase + bit_of(desc[sg.desc0 + b0].a[side], first_pair) + ngs * 64;
-    const uint64_t v = tile[pk][gl];
+    const uint64_t dst = secbase + bit_of(desc[sg.desc0 + b0].a[side], first_pair) + ngc * 64;
+    const uint64_t v = tile[pk][col];
     if ((dst & 63) == 0) {
         blob[dst >> 6] = v;  // inside the run: no other writer
     } else {
