@@ -479,6 +479,7 @@ struct SegList {
     struct Lane {
         uint32_t item;
         uint64_t chunks, desc, row0, row_words;
+        uint32_t narrow_out;
     };
     void add_lane_group(const std::vector<Lane>& group, uint32_t lane_g) {
         blocks = round8(blocks);
@@ -498,6 +499,7 @@ struct SegList {
             s.lane = static_cast<uint32_t>(k);
             s.lanes = static_cast<uint32_t>(group.size());
             s.lane_g = lane_g;
+            s.narrow_out = group[k].narrow_out;
             segs.push_back(s);
         }
         blocks += group_blocks;
@@ -607,6 +609,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         struct HbmItem {
             uint32_t item;
             uint64_t chunks, desc0, row0, row_words;
+            uint32_t narrow_out;  // the children are coloured in LDS: 16-bit entries
         };
         std::vector<HbmItem> hbm_items;
         uint64_t desc_count = desc_base, row_count = 0;
@@ -620,7 +623,9 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                          false, d);
             if (cmax > kLevelLds) {
                 const uint64_t chunks = ceil_div(maxpairs + 1, kBlock);
-                hbm_items.push_back(HbmItem{i, chunks, desc_count, row_count, row_words});
+                const uint32_t narrow_out = ceil_div(jobs[i].n, nodes * 2) <= kLevelLds ? 1u : 0u;
+                hbm_items.push_back(
+                    HbmItem{i, chunks, desc_count, row_count, row_words, narrow_out});
                 const uint64_t bin_chunks =
                     ceil_div(maxpairs + 1, uint64_t{kHookBinThreads} * kHookBinPairs);
                 bin.add(i, nodes * bin_chunks, nodes, bin_chunks, desc_count, row_count, row_words,
@@ -695,7 +700,8 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
                         while (lane_g < 64 && bytes * lane_g * 2 <= kPlainUnionGroupBytes) lane_g *= 2;
                     }
                     group.push_back(SegList::Lane{h.item, h.chunks, h.desc0 + j,
-                                                  h.row0 + j * h.row_words, h.row_words});
+                                                  h.row0 + j * h.row_words, h.row_words,
+                                                  h.narrow_out});
                     if (group.size() == uint64_t{8} * lane_g) {
                         hbm.add_lane_group(group, lane_g);
                         group.clear();
@@ -707,6 +713,9 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
             for (const HbmItem& h : hbm_items) {
                 hbm.add(h.item, nodes * h.chunks, nodes, h.chunks, h.desc0, h.row0, h.row_words,
                         true);
+                if (!hbm.segs.empty() && hbm.segs.back().item == h.item) {
+                    hbm.segs.back().narrow_out = h.narrow_out;
+                }
             }
         }
         lp.hbm = hbm.commit(table);

@@ -103,7 +103,10 @@ struct Seg {
     uint32_t lane;
     uint32_t lanes;
     uint32_t lane_g;
-    uint32_t pad;
+    // Level launches: the item's children at this depth are coloured in LDS
+    // (at most 2^16 entries each), so they are stored as 16-bit entries
+    // (narrow_at).
+    uint32_t narrow_out;
 };
 
 // Index of the segment of block b: the last one whose first block is <= b.
@@ -1084,6 +1087,36 @@ __device__ __forceinline__ uint64_t load_perm(const ItemCtx& it, const uint32_t*
     return level0 ? load_src(it, abs - it.base) : static_cast<uint64_t>(__builtin_nontemporal_load(lev + abs));
 }
 
+// A node of at most 2^16 entries (coloured by k_level_lds or k_subtree)
+// keeps its level arrays as 16-bit entries: entry i of the node at offset
+// `off` is 16-bit element 2 * off + i of the array, inside the node's own
+// range of 4-byte entries, so that nodes of both widths (items of different
+// sizes) share one array.  It halves what those levels stream.
+__device__ __forceinline__ uint16_t* narrow_at(uint32_t* lev, uint64_t off) {
+    return reinterpret_cast<uint16_t*>(lev) + 2 * off;
+}
+__device__ __forceinline__ const uint16_t* narrow_at(const uint32_t* lev, uint64_t off) {
+    return reinterpret_cast<const uint16_t*>(lev) + 2 * off;
+}
+
+// Stores the two children's entries of pair p (and the odd tail entry,
+// child 0's last) of the node at `off` with `lc` entries in child 0.
+__device__ __forceinline__ void store_children(uint32_t* dst, bool narrow, uint64_t off,
+                                               uint64_t lc, uint64_t p, uint32_t c0, uint32_t c1) {
+    if (narrow) {
+        __builtin_nontemporal_store(static_cast<uint16_t>(c0), narrow_at(dst, off) + p);
+        __builtin_nontemporal_store(static_cast<uint16_t>(c1), narrow_at(dst, off + lc) + p);
+    } else {
+        __builtin_nontemporal_store(c0, dst + off + p);
+        __builtin_nontemporal_store(c1, dst + off + lc + p);
+    }
+}
+__device__ __forceinline__ void store_tail(uint32_t* dst, bool narrow, uint64_t off, uint64_t h,
+                                           uint32_t v) {
+    if (narrow) narrow_at(dst, off)[h] = static_cast<uint16_t>(v);
+    else dst[off + h] = v;
+}
+
 // One thread per input pair: canonical switch value s_p (its slot's parity to
 // the root), child routing of P, and s ballot-packed into the node's row.
 // With a nonzero `group` (depths whose nodes keep to one XCD) each XCD takes
@@ -1134,10 +1167,12 @@ __global__ __launch_bounds__(kBlock) void k_route_inputs(const ItemCtx* __restri
             const W r = uf_find<DeviceForest>(u, x0, s);
             if (r != x0) DeviceForest::store(u + x0, (r << 1) | static_cast<W>(s));
         }
-        __builtin_nontemporal_store(static_cast<uint32_t>((s ? a1 : a0) >> 1), pdst + off + p);
-        __builtin_nontemporal_store(static_cast<uint32_t>((s ? a0 : a1) >> 1), pdst + off + lc + p);
+        store_children(pdst, sg.narrow_out != 0, off, lc, p,
+                       static_cast<uint32_t>((s ? a1 : a0) >> 1),
+                       static_cast<uint32_t>((s ? a0 : a1) >> 1));
     } else if ((cnt & 1) && p == h) {
-        pdst[off + h] = static_cast<uint32_t>(load_perm(it, plev, level0, off + cnt - 1) >> 1);
+        store_tail(pdst, sg.narrow_out != 0, off, h,
+                   static_cast<uint32_t>(load_perm(it, plev, level0, off + cnt - 1) >> 1));
     }
     const uint64_t ms = __ballot(s != 0);
     const uint64_t first = p - lane_id();
@@ -1178,8 +1213,9 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict_
         const uint64_t a1 = static_cast<uint64_t>(__builtin_nontemporal_load(inv + off + 2 * q + 1));
         // colour(x) = s_{x>>1} ^ (x & 1); the odd tail entry has colour 0.
         if (!((cnt & 1) && a0 == cnt - 1)) t = row_bit(srows, row, a0 >> 1) ^ (a0 & 1);
-        __builtin_nontemporal_store(static_cast<uint32_t>((t ? a1 : a0) >> 1), idst + off + q);
-        __builtin_nontemporal_store(static_cast<uint32_t>((t ? a0 : a1) >> 1), idst + off + lc + q);
+        store_children(idst, sg.narrow_out != 0, off, lc, q,
+                       static_cast<uint32_t>((t ? a1 : a0) >> 1),
+                       static_cast<uint32_t>((t ? a0 : a1) >> 1));
         if (next_uf) {
             const uint64_t lslots = (lc + 1) >> 1, rslots = (h + 1) >> 1;
             if (next_wide) {
@@ -1193,7 +1229,8 @@ __global__ __launch_bounds__(kBlock) void k_route_outputs(const Seg* __restrict_
             }
         }
     } else if ((cnt & 1) && q == h) {
-        idst[off + h] = static_cast<uint32_t>(static_cast<uint64_t>(inv[off + cnt - 1]) >> 1);
+        store_tail(idst, sg.narrow_out != 0, off, h,
+                   static_cast<uint32_t>(static_cast<uint64_t>(inv[off + cnt - 1]) >> 1));
     }
     const uint64_t mt = __ballot(t != 0);
     const uint64_t first = q - lane_id();
@@ -1402,10 +1439,12 @@ __global__ __launch_bounds__(kBlock) void k_ruler_route(const ItemCtx* __restric
         }
         const uint64_t a0 = load_perm(it, plev, level0, off + 2 * p);
         const uint64_t a1 = load_perm(it, plev, level0, off + 2 * p + 1);
-        pdst[off + p] = static_cast<uint32_t>((s ? a1 : a0) >> 1);
-        pdst[off + lc + p] = static_cast<uint32_t>((s ? a0 : a1) >> 1);
+        store_children(pdst, sg.narrow_out != 0, off, lc, p,
+                       static_cast<uint32_t>((s ? a1 : a0) >> 1),
+                       static_cast<uint32_t>((s ? a0 : a1) >> 1));
     } else if ((cnt & 1) && p == h) {
-        pdst[off + h] = static_cast<uint32_t>(load_perm(it, plev, level0, off + cnt - 1) >> 1);
+        store_tail(pdst, sg.narrow_out != 0, off, h,
+                   static_cast<uint32_t>(load_perm(it, plev, level0, off + cnt - 1) >> 1));
     }
     const uint64_t ms = __ballot(s != 0);
     const uint64_t first = p - lane_id();
@@ -1453,9 +1492,24 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
     for (uint32_t i = tid; i < (lc + 1) / 2; i += nthreads) uf_words[i] = 0xFFFFFFFFu;
     if (tid == 0) ndeferred = 0;
     __syncthreads();
+    // Below depth 0 the node's P and I are 16-bit entries (narrow_at): a
+    // pair is one 32-bit load.
+    const uint32_t* const ipairs =
+        reinterpret_cast<const uint32_t*>(narrow_at(reinterpret_cast<const uint32_t*>(isrc), off));
+    const uint32_t* const ppairs = reinterpret_cast<const uint32_t*>(narrow_at(plev, off));
+    auto ipair = [&](uint32_t q, uint32_t& a, uint32_t& b) {
+        if (level0) {
+            a = static_cast<uint32_t>(isrc[off + 2 * q]);
+            b = static_cast<uint32_t>(isrc[off + 2 * q + 1]);
+        } else {
+            const uint32_t two = __builtin_nontemporal_load(ipairs + q);
+            a = two & 0xFFFFu;
+            b = two >> 16;
+        }
+    };
     for (uint32_t q = tid; q < h; q += nthreads) {
-        const uint32_t a = static_cast<uint32_t>(isrc[off + 2 * q]);
-        const uint32_t b = static_cast<uint32_t>(isrc[off + 2 * q + 1]);
+        uint32_t a, b;
+        ipair(q, a, b);
         const uint16_t sa = static_cast<uint16_t>(in_slot(a, cnt));
         const uint16_t sb = static_cast<uint16_t>(in_slot(b, cnt));
         if (sa == sb) continue;
@@ -1492,13 +1546,21 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
         const uint32_t p = base + lane;
         uint32_t s = 0;
         if (p < h) {
+            uint32_t a0, a1;
+            if (level0) {
+                a0 = static_cast<uint32_t>(load_src(it, off + 2 * p - it.base));
+                a1 = static_cast<uint32_t>(load_src(it, off + 2 * p + 1 - it.base));
+            } else {
+                const uint32_t two = __builtin_nontemporal_load(ppairs + p);
+                a0 = two & 0xFFFFu;
+                a1 = two >> 16;
+            }
             uf_find<GroupForest>(uf, static_cast<uint16_t>(p + odd), s);
-            const uint64_t a0 = load_perm(it, plev, level0, off + 2 * p);
-            const uint64_t a1 = load_perm(it, plev, level0, off + 2 * p + 1);
-            pdst[off + p] = static_cast<uint32_t>((s ? a1 : a0) >> 1);
-            pdst[off + lc + p] = static_cast<uint32_t>((s ? a0 : a1) >> 1);
+            store_children(pdst, true, off, lc, p, (s ? a1 : a0) >> 1, (s ? a0 : a1) >> 1);
         } else if (odd && p == h) {
-            pdst[off + h] = static_cast<uint32_t>(load_perm(it, plev, level0, off + cnt - 1) >> 1);
+            const uint32_t v = level0 ? static_cast<uint32_t>(load_src(it, off + cnt - 1 - it.base))
+                                      : narrow_at(plev, off)[cnt - 1];
+            store_tail(pdst, true, off, h, v >> 1);
         }
         const uint64_t ms = __ballot(s != 0);
         if (lane == 0) {
@@ -1512,16 +1574,18 @@ __global__ __launch_bounds__(kLevelThreads) void k_level_lds(
         const uint32_t q = base + lane;
         uint32_t t = 0;
         if (q < h) {
-            const uint32_t a0 = static_cast<uint32_t>(isrc[off + 2 * q]);
-            const uint32_t a1 = static_cast<uint32_t>(isrc[off + 2 * q + 1]);
+            uint32_t a0, a1;
+            ipair(q, a0, a1);
             if (!(odd && a0 == cnt - 1)) {
                 const uint32_t pa = a0 >> 1;
                 t = static_cast<uint32_t>((sbits[pa >> 6] >> (pa & 63)) & 1) ^ (a0 & 1);
             }
-            idst[off + q] = (t ? a1 : a0) >> 1;
-            idst[off + lc + q] = (t ? a0 : a1) >> 1;
+            store_children(idst, true, off, lc, q, (t ? a1 : a0) >> 1, (t ? a0 : a1) >> 1);
         } else if (odd && q == h) {
-            idst[off + h] = static_cast<uint32_t>(isrc[off + cnt - 1]) >> 1;
+            const uint32_t v =
+                level0 ? static_cast<uint32_t>(isrc[off + cnt - 1])
+                       : narrow_at(reinterpret_cast<const uint32_t*>(isrc), off)[cnt - 1];
+            store_tail(idst, true, off, h, v >> 1);
         }
         const uint64_t mt = __ballot(t != 0);
         if (lane == 0 && base < h) trows[row + (base >> 6)] = mt;
@@ -2218,11 +2282,12 @@ __global__ __launch_bounds__(Threads) __attribute__((amdgpu_waves_per_eu(
                 v[j] = i < n0 ? static_cast<uint32_t>(load_src(it, base + i)) : 0u;
             }
         } else {
-            const uint32_t* src = plev + it.base + base;
+            // 16-bit entries (narrow_at), as every node of at most 2^16.
+            const uint16_t* src = narrow_at(plev, it.base + base);
 #pragma unroll
             for (uint32_t j = 0; j < kPer; ++j) {
                 const uint32_t i = tid + j * Threads;
-                v[j] = i < n0 ? __builtin_nontemporal_load(src + i) : 0u;
+                v[j] = i < n0 ? static_cast<uint32_t>(__builtin_nontemporal_load(src + i)) : 0u;
             }
         }
 #pragma unroll
