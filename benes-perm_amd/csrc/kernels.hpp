@@ -259,10 +259,16 @@ __global__ void k_headers(const ItemCtx* __restrict__ items, uint32_t count,
 // of its own.  Blocks of one XCD then either take its nodes one after
 // another (`spread` false: the blocks in flight share one node, whose words
 // stay in L2), or visit its nodes in turn (`spread` true: the blocks in flight
-// cover up to kSpreadNodes of them).  The union pass spreads: its compare-and-swaps go past
-// L2 to memory, and crowding them on one node's words serialises them there.
+// cover up to kSpreadNodes of them, and no more than kSpreadChunks chunks'
+// worth of nodes, about 16 MB of words).  The union pass spreads: its
+// compare-and-swaps go past L2 to memory, and crowding them on one small
+// node's words serialises them there; nodes of 16 MB or more are taken one at
+// a time, since more of them in flight only widen the span of random accesses
+// (at 2^32 + 12345 entries, depths 4-9: 1000 ms with 64 nodes per XCD, 857
+// with one).
 constexpr uint32_t kXcds = 8;
-constexpr uint64_t kSpreadNodes = 64;
+constexpr uint32_t kSpreadNodes = 64;
+constexpr uint32_t kSpreadChunks = 1u << 14;
 
 __device__ __forceinline__ bool block_to_node_at(const Seg& sg, uint64_t block, uint64_t& node,
                                                  uint64_t& chunk, bool spread) {
@@ -283,7 +289,11 @@ __device__ __forceinline__ bool block_to_node_at(const Seg& sg, uint64_t block, 
             // At most kSpreadNodes of the XCD's nodes at a time, so that very
             // large items (thousands of nodes per XCD) keep the live words
             // within reach of the address translation caches.
-            const uint32_t g = per_xcd < kSpreadNodes ? per_xcd : static_cast<uint32_t>(kSpreadNodes);
+            // Group sizes are powers of two, as per_xcd is, so that the
+            // groups tile the XCD's nodes exactly.
+            uint32_t cap = kSpreadNodes;
+            while (cap > 1 && cap * chunks > kSpreadChunks) cap >>= 1;
+            const uint32_t g = per_xcd < cap ? per_xcd : cap;
             const uint32_t grp = local / (g * chunks), in_grp = local - grp * (g * chunks);
             n = (grp * g + in_grp % g) * kXcds + xcd;
             c = in_grp / g;
