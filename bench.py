@@ -67,7 +67,7 @@ def algorithmic_bytes(kernel: str, n: int, blob: int) -> float:
         "k_union_plain": 4.0 * n,       # inverse read (the same pass with plain-store hooks)
         "k_route_inputs": 10.0 * n + n / 8,   # P read, children written, slot words read, row bits
         "k_route_outputs": 8.0 * n + n / 4,   # I read, children written, row bits
-        "k_level_lds": 16.0 * n + n / 4,      # P and I read, both children written, row bits
+        "k_level_lds": 8.0 * n + n / 4,       # P and I read, both children written (16-bit entries), row bits
         "k_scatter_inverse": 8.0 * n,   # source read, inverse written
         "k_check_inverse": 4.0 * n,
         "k_pack": n / 8.0,              # one side's row bits of one depth, read and written
@@ -516,6 +516,8 @@ def main():
                     help="only run a plain 256 MiB device copy (HBM counter calibration)")
     ap.add_argument("--large-only", action="store_true",
                     help="only the 2^32+12345 u64 leg, printed as its own JSON line (profiling)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched leg, printed as its own JSON line (profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this backend compresses one permutation (or one "
@@ -529,6 +531,10 @@ def main():
     if args.large_only:
         assert B.available(), "gfx950 backend unavailable"
         print(json.dumps({"u64_2e32": large_u64()}), flush=True)
+        return
+    if args.batched_only:
+        assert B.available(), "gfx950 backend unavailable"
+        print(json.dumps({"batched": batched(args.batch_items)}), flush=True)
         return
     assert B.available(), "gfx950 backend unavailable"
     t0 = time.perf_counter()
