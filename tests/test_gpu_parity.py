@@ -113,6 +113,36 @@ def test_global_levels(n):
             assert B.compress(perm, S.fmt(g, pl, ic, oc)) == S.oracle_compress(perm, g, pl, ic, oc)
 
 
+@pytest.mark.parametrize("wide", [0, 1], ids=["words32", "words64"])
+def test_sixteen_bit_level_arrays(wide):
+    # Nodes of at most 2^16 entries keep their level arrays as 16-bit entries
+    # inside their own range of the shared 32-bit arrays; the last depth
+    # coloured in HBM writes its children that way only where every node of
+    # the item's next depth is that small.  Sizes on both sides of 2^16 and
+    # 2^17, one at a time and mixed in one batch (one depth then holds HBM
+    # nodes of some items and LDS nodes of others), with 32- and 64-bit
+    # union-find words and level-0 inverse.
+    sizes = [1 << 16, (1 << 16) + 1, (1 << 17) - 1, 1 << 17, (1 << 17) + 1, (1 << 18) + 7]
+    rng = np.random.default_rng(0x16B)
+    B.set_wide_words(wide)
+    try:
+        perms, fmts, want = [], [], []
+        for i, n in enumerate(sizes):
+            width = 8 if i % 2 else 4
+            perm = rng.permutation(n).astype(np.uint64 if width == 8 else np.uint32)
+            g, pl = (5, 0) if i % 3 else (4, 1)
+            ref = S.oracle_compress(perm.astype(np.uint64), g, pl)
+            print(f"16-bit levels wide={wide} n={n} width={width}", flush=True)
+            assert B.compress(perm, S.fmt(g, pl)) == ref, n
+            perms.append(perm)
+            fmts.append(S.fmt(g, pl))
+            want.append(ref)
+        print(f"16-bit levels wide={wide} batch of {len(perms)}", flush=True)
+        assert B.compress_batch(perms, fmts) == want
+    finally:
+        B.set_wide_words(0)
+
+
 @pytest.mark.parametrize("n", [(1 << 19) - 3, (1 << 20) + 12345])
 def test_tall_cluster_groups(n):
     # One cluster group spanning most levels: near its root a packet's share
