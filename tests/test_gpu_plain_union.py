@@ -76,6 +76,10 @@ def test_random_matches_oracle(mode, n, g, pl, ic, oc, width):
     print(f"union mode={mode} random n={n} g={g} pl={pl} width={width}", flush=True)
     got, stats = compress_profiled(perm, S.fmt(g, pl, ic, oc))
     assert_union_kernels(stats, mode, local_depths(n))
+    if mode == 1 and n >= (1 << 22):
+        # Millions of plain-store hooks race on shared roots: some relations
+        # are overwritten and joined again (about one hook in 70 at 2^26).
+        assert stats.get("union_rejoined", (0, 0))[1] > 0, stats
     assert got == S.oracle_compress(perm, g, pl, ic, oc)
 
 
