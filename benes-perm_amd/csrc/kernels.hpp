@@ -854,8 +854,8 @@ __global__ __launch_bounds__(kBlock) void k_union(const Seg* __restrict__ segs, 
 // store to it, and only the later relation stays.  Each thread therefore
 // records the hook it made, (hooked slot << 32) | word, at
 // rec[(off + 2q) >> 1] (distinct for every pair of every node), or
-// kNoHook; k_union_verify then joins again the relations that were
-// overwritten.  Every stored word is a true relation to a smaller slot, so
+// kNoHook; k_union_verify then finds the relations that were overwritten
+// and k_union_rejoin joins them again.  Every stored word is a true relation to a smaller slot, so
 // the walks of this pass stay correct whatever mix of old and new words they
 // read; a walk that relied on an overwritten relation is covered once that
 // relation is joined again, since the relations it followed are facts.
