@@ -454,7 +454,7 @@ struct SegList {
     uint64_t blocks = 0;
     void add(uint32_t item, uint64_t blocks_of_item, uint64_t nodes, uint64_t chunks,
              uint64_t desc0, uint64_t row0, uint64_t row_words, bool xcd_align,
-             uint32_t depth = 0, uint32_t side = 0) {
+             uint32_t depth = 0, uint32_t side = 0, uint32_t narrow_out = 0) {
         if (blocks_of_item == 0) return;
         if (xcd_align) {
             blocks = round8(blocks);
@@ -471,6 +471,7 @@ struct SegList {
         s.item = item;
         s.depth = depth;
         s.side = side;
+        s.narrow_out = narrow_out;
         segs.push_back(s);
         blocks += blocks_of_item;
     }
@@ -712,10 +713,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         } else {
             for (const HbmItem& h : hbm_items) {
                 hbm.add(h.item, nodes * h.chunks, nodes, h.chunks, h.desc0, h.row0, h.row_words,
-                        true);
-                if (!hbm.segs.empty() && hbm.segs.back().item == h.item) {
-                    hbm.segs.back().narrow_out = h.narrow_out;
-                }
+                        true, 0, 0, h.narrow_out);
             }
         }
         lp.hbm = hbm.commit(table);
