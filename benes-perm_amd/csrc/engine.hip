@@ -814,7 +814,7 @@ void enqueue_batch(Workspace& ws, hipStream_t st, std::vector<Job>& jobs, Profil
         ws.trows.ensure(8 * max_rows);
         ws.deferred.ensure(sizeof(DeferredEdge) * defer_cap);
         // Per depth: 4 counters, then (after all depths) kRejoinCounters
-        // counters of the relations k_union_verify joined again (counted
+        // counters of the relations k_union_rejoin joined again (counted
         // when profiling).
         ws.deferred_counts.ensure(sizeof(unsigned long long) * (4 + kRejoinCounters) * max_df);
         // One count per check block of the plain-store union pass.
